@@ -1,0 +1,135 @@
+// Shared device/host helpers for the gfx950 render core.
+//
+// Float semantics: the library is built with -ffp-contract=off and HIP's
+// default correctly rounded f32 division and square root, so every
+// expression below rounds exactly like the scalar IEEE restatement of the
+// reference.  Vector operators keep the reference's evaluation order
+// (src/util/math.h:91-246); normalize is v * (1/sqrtf(v.v)) rather than the
+// reference's device rsqrtf (src/util/math.h:210-211), whose approximation
+// is not specified.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aur {
+
+// src/util/math.h:17-22
+constexpr float kInfinity = 1000000.0f;
+constexpr float kEpsilon = 0.0000001f;
+constexpr float kPi = 3.1415926536f;
+constexpr float kInvPi = 0.3183098862f;
+
+struct V3 {
+  float x, y, z;
+};
+
+__host__ __device__ inline V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__host__ __device__ inline V3 mk(float v) { return V3{v, v, v}; }
+__host__ __device__ inline V3 ld3(const float* p) { return V3{p[0], p[1], p[2]}; }
+__host__ __device__ inline V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline V3 operator*(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline V3 operator*(V3 a, float b) { return V3{a.x * b, a.y * b, a.z * b}; }
+__host__ __device__ inline V3 operator*(float a, V3 b) { return V3{a * b.x, a * b.y, a * b.z}; }
+__host__ __device__ inline V3 operator/(V3 a, float b) { return V3{a.x / b, a.y / b, a.z / b}; }
+__host__ __device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline float length(V3 v) { return sqrtf(dot(v, v)); }
+__host__ __device__ inline V3 normalize(V3 v) { return v * (1.0f / sqrtf(dot(v, v))); }
+__host__ __device__ inline V3 bclerp(V3 a, V3 b, V3 c, float u, float v) {
+  return (1.0f - u - v) * a + u * b + v * c;
+}
+__host__ __device__ inline V3 reflect(V3 i, V3 n) { return i - 2.0f * n * dot(n, i); }
+__host__ __device__ inline bool refract(V3& r, V3 i, V3 n, float eta1, float eta2) {
+  const float eta = eta1 / eta2;
+  const float cosi = dot(n, i);
+  const float sint = eta * eta * (1.0f - cosi * cosi);
+  if (sint > 1.0f) return false;
+  r = eta * i - (eta + sqrtf(1.0f - sint)) * n;
+  return true;
+}
+__host__ __device__ inline bool is_zero(V3 v) {
+  return v.x < kEpsilon && v.y < kEpsilon && v.z < kEpsilon;
+}
+__host__ __device__ inline float comp(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// Order-preserving map of a non-NaN float (with -0.0 already folded to +0.0)
+// onto uint32: a < b as floats iff key(a) < key(b).  Never returns 0.
+__host__ __device__ inline uint32_t fkey(float f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t u = __float_as_uint(f);
+#else
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+#endif
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the
+// multiply/xor round function with the published constants.
+struct U4 {
+  uint32_t x, y, z, w;
+};
+__host__ __device__ inline U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c.x;
+    const uint64_t p1 = (uint64_t)M1 * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+__host__ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// RNG stream keys: compat mode draws sequentially per pixel sample; PBR mode
+// addresses (pixel, sample, bounce, slot).
+constexpr uint32_t kKeyCompat = 0x2545F491u;
+constexpr uint32_t kKeyPbr = 0x6A09E667u;
+constexpr uint32_t kTagPbr = 0x50425200u;
+
+// Traversal node record: the two triangles of node k as (v1, e1 = v2-v1,
+// e2 = v3-v1) -- the same float values Ray::intersect computes on the fly
+// (src/kernels/lib/ray.cuh:25-26) -- followed by the slabs of the node's two
+// children on the children's axis (min/max of their six axis coordinates,
+// src/kernels/lib/intersect.cuh:106-111).  96 bytes, 16-byte aligned.
+struct alignas(16) NodeRec {
+  float tri[2][9];
+  float cslab[4];  // left lo, left hi, right lo, right hi
+  float pad[2];
+};
+static_assert(sizeof(NodeRec) == 96, "node record");
+
+// Subtree arithmetic of a left-complete binary tree of n nodes in heap order.
+// Number of nodes in the subtrees of the consecutive same-depth range [a, b).
+__host__ __device__ inline uint64_t desc_count(uint64_t a, uint64_t b, uint64_t n) {
+  uint64_t total = 0;
+  uint64_t lo = a, hi = b;  // range at the current relative depth
+  while (lo < n) {
+    total += (hi < n ? hi : n) - lo;
+    lo = 2 * lo + 1;
+    hi = 2 * hi + 1;
+  }
+  return total;
+}
+
+__host__ __device__ inline uint32_t depth_of(uint32_t k) {
+  uint32_t d = 0;
+  uint32_t x = k + 1;
+  while (x > 1) {
+    x >>= 1;
+    ++d;
+  }
+  return d;
+}
+
+}  // namespace aur

@@ -1,0 +1,122 @@
+// Internal state of an aur_context and the device-buffer helper.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <utility>
+
+#include "../../include/aurora_amd.h"
+#include "common.h"
+
+namespace aur {
+
+// Owning device allocation (hipMalloc / hipFree).
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Ensures capacity for `count` elements; contents are not preserved.
+  hipError_t reserve(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    if (count == 0) return hipSuccess;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+    p = static_cast<T*>(q);
+    n = count;
+    return hipSuccess;
+  }
+};
+
+struct FrameState {
+  uint32_t width = 0, height = 0, spp = 0;
+  DevBuf<float4> pixels;  // RGBA accumulation (framebuffer)
+  // PBR wavefront state (one path per pixel per sample pass)
+  DevBuf<float4> ray_o, ray_d, beta, radiance;
+  DevBuf<aur_hit> hits;
+  DevBuf<uint32_t> queue[2];
+  DevBuf<uint32_t> counters;  // [0..1] queue lengths, [2] shadow count, [3..] stats
+  DevBuf<float4> sh_o, sh_d, sh_c;  // shadow rays: origin+tmax, dir, contribution
+  DevBuf<int32_t> sh_occ;
+  size_t shadow_cap = 0;
+  uint64_t rays_traced = 0;
+};
+
+}  // namespace aur
+
+struct aur_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  // uploaded geometry (input order)
+  uint32_t count = 0;
+  aur::DevBuf<float> in_vertices, in_normals, in_texcoords;
+  aur::DevBuf<uint16_t> in_shader_ids;
+
+  // tree-order geometry and traversal records (valid after the build)
+  bool built = false;
+  bool has_tb = false;
+  uint32_t levels = 0;
+  aur::DevBuf<float> vertices, normals, texcoords, tangents, bitangents;
+  aur::DevBuf<uint16_t> shader_ids;
+  aur::DevBuf<uint32_t> perm;
+  aur::DevBuf<aur::NodeRec> nodes;
+  float root_lo = 0.0f, root_hi = 0.0f;
+
+  // materials and lights
+  aur::DevBuf<aur_shader> shaders;
+  uint32_t num_shaders = 0;
+  aur::DevBuf<aur_light> lights;
+  uint32_t num_lights = 0;
+
+  aur::FrameState frame;
+
+  aur_status fail(aur_status code, const std::string& msg) {
+    last_error = msg;
+    return code;
+  }
+  aur_status hip_fail(hipError_t e, const char* what) {
+    last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? AUR_ERR_OOM : AUR_ERR_DEVICE;
+  }
+};
+
+#define AUR_HIP(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return (ctx)->hip_fail(_e, #expr); \
+  } while (0)
+
+#define AUR_LAUNCH_CHECK(ctx, what)                          \
+  do {                                                       \
+    hipError_t _e = hipGetLastError();                       \
+    if (_e != hipSuccess) return (ctx)->hip_fail(_e, what);  \
+  } while (0)
+
+// Entry points implemented across translation units.
+namespace aur {
+aur_status nmh_build(aur_context* ctx, aur_build_stats* stats);
+aur_status generate_tb(aur_context* ctx);
+aur_status trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits);
+aur_status trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* ids);
+aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
+                               uint64_t* popped, uint64_t* passed);
+aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
+                        uint32_t s0, uint32_t s1);
+}  // namespace aur

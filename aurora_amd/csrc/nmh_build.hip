@@ -1,0 +1,672 @@
+// NMH build on gfx950: replaces cudaRebuildNMH (src/kernels/rebuilder.cu:311-450).
+//
+// Result (bit-identical to the reference's canonical semantics): the tree
+// order of the triangles.  Node k of the implicit heap owns tree positions
+// 2k and 2k+1; its children are nodes 2k+1 and 2k+2.
+//
+// The reference sorts the whole array by (node label, min key) at every level
+// (sortTriangles, rebuilder.cu:262-289) with two Thrust radix sorts, swaps the
+// first maximum of the max key into slot +1 (rebuilder.cu:127-151), relabels
+// nodes and emits child splits through a host round trip per level
+// (rebuilder.cu:247-260).  Here:
+//   * subtree sizes and offsets of the left-complete tree are closed forms
+//     (desc_count), so split emission and all host synchronisation vanish;
+//   * only the pending region is sorted; finished node pairs are not moved
+//     again -- their final in-pair order, which the reference obtains by
+//     re-sorting them at every later level, is computed once as a
+//     lexicographic comparison of the later levels' keys (newest first; only
+//     the last three levels can differ since axes cycle with period 3);
+//   * while node groups are larger than one workgroup's LDS tile, a level is a
+//     segmented stable LSD radix sort (4 x 8-bit digits, tile histograms,
+//     per-group digit offsets, wave-ballot stable ranking) followed by a
+//     tiled first-maximum reduction and an emit pass that writes finished
+//     pairs and compacts the remainders for the next level;
+//   * once every group fits (<= kSubT elements) one workgroup takes a whole
+//     subtree and runs all remaining levels in LDS with a bitonic sort on
+//     (group, key, position) composite keys.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "context.h"
+
+namespace aur {
+namespace {
+
+constexpr uint32_t kTile = 2048;  // radix / emit tile (elements)
+constexpr uint32_t kSubT = 2048;  // max elements of an LDS-resident subtree
+constexpr int kThreads = 256;
+
+struct TileDesc {
+  uint32_t start, count, child, group;
+};
+struct ChildDesc {
+  uint32_t off, size, tb, te;
+};
+struct GroupDesc {
+  uint32_t off, size, tb, te;
+};
+struct SubGroup {
+  uint32_t off, size, first, count;
+};
+
+__device__ inline void write_pair(uint32_t h, uint32_t x, uint32_t y, int level, int L,
+                                  const float* __restrict__ bmin, uint32_t N,
+                                  uint32_t* __restrict__ P) {
+  bool y_first = false;
+  const int lo = max(level + 1, L - 3);
+  for (int lev = L - 1; lev >= lo; --lev) {
+    const float* km = bmin + (size_t)(lev % 3) * N;
+    const float kx = km[x], ky = km[y];
+    if (ky < kx) {
+      y_first = true;
+      break;
+    }
+    if (kx < ky) break;
+  }
+  P[2 * (size_t)h] = y_first ? y : x;
+  P[2 * (size_t)h + 1] = y_first ? x : y;
+}
+
+// computeTriangleBounds (rebuilder.cu:72-91), SoA [axis][triangle].
+__global__ void k_bounds(const float* __restrict__ v, uint32_t N, float* __restrict__ bmin,
+                         float* __restrict__ bmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float* p = v + (size_t)i * 9;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    bmin[(size_t)a * N + i] = fminf(fminf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]);
+    bmax[(size_t)a * N + i] = fmaxf(fmaxf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]);
+  }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ W, uint32_t N) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) W[i] = i;
+}
+
+// Indices read back from device arrays are clamped to [0, N) before they
+// address another array: a defect then shows up as a wrong tree in the parity
+// tests, never as an out-of-bounds access.
+__device__ __forceinline__ uint32_t clampi(uint32_t i, uint32_t N) { return i < N ? i : N - 1; }
+
+__global__ void k_fill0(uint32_t* __restrict__ p, uint32_t N) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) p[i] = 0;
+}
+
+__global__ void k_gather_keys(const uint32_t* __restrict__ W, uint32_t m, uint32_t N,
+                              const float* __restrict__ keys, uint32_t* __restrict__ K) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) K[i] = fkey(keys[clampi(W[i], N)]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restrict__ tiles,
+                                                         const uint32_t* __restrict__ K,
+                                                         uint32_t shift,
+                                                         uint32_t* __restrict__ H) {
+  __shared__ uint32_t h[256];
+  const TileDesc t = tiles[blockIdx.x];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < t.count; i += kThreads)
+    atomicAdd(&h[(K[t.start + i] >> shift) & 255u], 1u);
+  __syncthreads();
+  H[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// Per group: digit-major, tile-minor exclusive offsets (stable scatter).
+__global__ __launch_bounds__(kThreads) void k_radix_offsets(const GroupDesc* __restrict__ groups,
+                                                            const uint32_t* __restrict__ H,
+                                                            uint32_t* __restrict__ O) {
+  __shared__ uint32_t s[256];
+  const GroupDesc g = groups[blockIdx.x];
+  const uint32_t d = threadIdx.x;
+  uint32_t total = 0;
+  for (uint32_t t = g.tb; t < g.te; ++t) total += H[(size_t)t * 256 + d];
+  s[d] = total;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = d >= off ? s[d - off] : 0u;
+    __syncthreads();
+    s[d] += v;
+    __syncthreads();
+  }
+  uint32_t run = g.off + s[d] - total;
+  for (uint32_t t = g.tb; t < g.te; ++t) {
+    O[(size_t)t * 256 + d] = run;
+    run += H[(size_t)t * 256 + d];
+  }
+}
+
+// Stable scatter: elements of a tile are ranked in order within their digit.
+// Rounds of 256 elements; inside a wave, lanes with the same digit are found
+// with 8 ballots; per-wave digit counts go through LDS in wave order.
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+    const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
+    const uint32_t* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ O, uint32_t m,
+    uint32_t* __restrict__ Kout, uint32_t* __restrict__ Vout) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[kThreads / 64][256];
+  const TileDesc t = tiles[blockIdx.x];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  base[tid] = O[(size_t)blockIdx.x * 256 + tid];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t r0 = 0; r0 < t.count; r0 += kThreads) {
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t i = r0 + tid;
+    const bool valid = i < t.count;
+    const uint32_t k = valid ? Kin[t.start + i] : 0u;
+    const uint32_t v = valid ? Vin[t.start + i] : 0u;
+    const uint32_t dg = (k >> shift) & 255u;
+    uint64_t mask = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (dg >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      mask &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = __popcll(mask & lt_mask);
+    if (valid && rank == 0) wcnt[wave][dg] = __popcll(mask);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base[dg] + rank;
+      for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][dg];
+      if (pos < m) {
+        Kout[pos] = k;
+        Vout[pos] = v;
+      }
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) add += wcnt[w][tid];
+    base[tid] += add;
+    __syncthreads();
+  }
+}
+
+// First maximum of the max key over child positions [off+1, off+size)
+// restricted to this tile: (key, position), key 0 = none.
+__global__ __launch_bounds__(kThreads) void k_argmax_partial(const TileDesc* __restrict__ tiles,
+                                                             const ChildDesc* __restrict__ children,
+                                                             const uint32_t* __restrict__ W,
+                                                             uint32_t N,
+                                                             const float* __restrict__ kmax,
+                                                             uint2* __restrict__ partial) {
+  __shared__ uint32_t sk[kThreads / 64], sp[kThreads / 64];
+  const TileDesc t = tiles[blockIdx.x];
+  const ChildDesc c = children[t.child];
+  const uint32_t lo = max(t.start, c.off + 1), hi = t.start + t.count;
+  uint32_t bk = 0, bp = 0xFFFFFFFFu;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) {
+    const uint32_t k = fkey(kmax[clampi(W[i], N)]);
+    if (k > bk) {
+      bk = k;
+      bp = i;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t ok = __shfl_xor(bk, off), op = __shfl_xor(bp, off);
+    if (ok > bk || (ok == bk && op < bp)) {
+      bk = ok;
+      bp = op;
+    }
+  }
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sk[wave] = bk;
+    sp[wave] = bp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w)
+      if (sk[w] > bk || (sk[w] == bk && sp[w] < bp)) {
+        bk = sk[w];
+        bp = sp[w];
+      }
+    partial[blockIdx.x] = make_uint2(bk, bp);
+  }
+}
+
+__global__ void k_argmax_combine(const ChildDesc* __restrict__ children, uint32_t nchildren,
+                                 const uint2* __restrict__ partial, uint32_t* __restrict__ argpos) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchildren) return;
+  const ChildDesc d = children[c];
+  uint32_t bk = 0, bp = 0xFFFFFFFFu;
+  for (uint32_t t = d.tb; t < d.te; ++t) {
+    const uint2 p = partial[t];
+    if (p.x > bk || (p.x == bk && p.y < bp)) {
+      bk = p.x;
+      bp = p.y;
+    }
+  }
+  argpos[c] = bp;
+}
+
+// Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders.
+__global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ tiles,
+                                                   const ChildDesc* __restrict__ children,
+                                                   const uint32_t* __restrict__ argpos,
+                                                   const uint32_t* __restrict__ W,
+                                                   uint32_t* __restrict__ Wnext, uint32_t first,
+                                                   int level, int L, const float* __restrict__ bmin,
+                                                   uint32_t N, uint32_t* __restrict__ P) {
+  const TileDesc t = tiles[blockIdx.x];
+  const ChildDesc c = children[t.child];
+  uint32_t am = argpos[t.child];
+  if (am <= c.off || am >= c.off + c.size) am = c.off + 1;  // cannot happen for sizes >= 2
+  for (uint32_t r = threadIdx.x; r < t.count; r += kThreads) {
+    const uint32_t i = t.start + r;
+    if (i == c.off) {
+      write_pair(first + t.child, W[c.off], W[am], level, L, bmin, N, P);
+    } else if (i >= c.off + 2) {
+      const uint32_t val = (i == am) ? W[c.off + 1] : W[i];
+      Wnext[i - 2 * (t.child + 1)] = val;
+    }
+  }
+}
+
+// One workgroup per group: all remaining levels of the group's subtree(s).
+__global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict__ groups,
+                                                      const uint32_t* __restrict__ W,
+                                                      const float* __restrict__ bmin,
+                                                      const float* __restrict__ bmax, uint32_t N,
+                                                      uint32_t n, int L, int level0,
+                                                      uint32_t* __restrict__ P) {
+  __shared__ uint32_t E[kSubT];
+  __shared__ uint32_t E2[kSubT];
+  __shared__ uint16_t G[kSubT];
+  __shared__ unsigned long long K[kSubT];
+  __shared__ uint32_t nodeOff[kSubT / 2 + 1];
+  __shared__ uint32_t nodeArg[kSubT / 2];
+  uint32_t* MK = reinterpret_cast<uint32_t*>(K);  // max keys reuse the sort buffer
+
+  const SubGroup g = groups[blockIdx.x];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  uint32_t m = g.size;
+  for (uint32_t i = tid; i < m; i += kThreads) {
+    E[i] = clampi(W[g.off + i], N);
+    G[i] = 0;
+  }
+  uint32_t f = g.first, c = g.count;
+  __syncthreads();
+  for (int level = level0; level < L && m > 0; ++level) {
+    const int a = level % 3;
+    const float* kmin = bmin + (size_t)a * N;
+    const float* kmax = bmax + (size_t)a * N;
+    for (uint32_t j = tid; j <= c; j += kThreads) nodeOff[j] = 2u * (uint32_t)desc_count(f, f + j, n);
+    uint32_t M = 1;
+    while (M < m) M <<= 1;
+    for (uint32_t i = tid; i < M; i += kThreads) {
+      K[i] = i < m ? (((unsigned long long)G[i] << 43) |
+                      ((unsigned long long)fkey(kmin[E[i]]) << 11) | i)
+                   : ~0ull;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= M; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t t = tid; t < M / 2; t += kThreads) {
+          const uint32_t i = 2 * j * (t / j) + (t % j);
+          const uint32_t ixj = i + j;
+          const bool up = (i & k) == 0;
+          const unsigned long long x = K[i], y = K[ixj];
+          if ((x > y) == up) {
+            K[i] = y;
+            K[ixj] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = tid; i < m; i += kThreads) E2[i] = E[(uint32_t)(K[i] & 0x7FFull)];
+    __syncthreads();
+    if (level == L - 1) {
+      for (uint32_t j = tid; j < c; j += kThreads) {
+        const uint32_t o = nodeOff[j];
+        P[2 * (size_t)(f + j)] = E2[o];
+        P[2 * (size_t)(f + j) + 1] = E2[o + 1];
+      }
+      break;
+    }
+    for (uint32_t i = tid; i < m; i += kThreads) MK[i] = fkey(kmax[E2[i]]);
+    __syncthreads();
+    for (uint32_t j = wave; j < c; j += kThreads / 64) {
+      const uint32_t o = nodeOff[j], e = nodeOff[j + 1];
+      uint32_t bk = 0, bp = 0xFFFFFFFFu;
+      for (uint32_t i = o + 1 + lane; i < e; i += 64) {
+        const uint32_t k = MK[i];
+        if (k > bk) {
+          bk = k;
+          bp = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t ok = __shfl_xor(bk, off), op = __shfl_xor(bp, off);
+        if (ok > bk || (ok == bk && op < bp)) {
+          bk = ok;
+          bp = op;
+        }
+      }
+      if (lane == 0) nodeArg[j] = (bp > o && bp < e) ? bp : o + 1;  // sizes are >= 2
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < c; j += kThreads)
+      write_pair(f + j, E2[nodeOff[j]], E2[nodeArg[j]], level, L, bmin, N, P);
+    for (uint32_t i = tid; i < m; i += kThreads) {
+      uint32_t lo = 0, hi = c;  // find j with nodeOff[j] <= i < nodeOff[j+1]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (nodeOff[mid] <= i)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const uint32_t o = nodeOff[lo];
+      if (i >= o + 2) {
+        const uint32_t val = (i == nodeArg[lo]) ? E2[o + 1] : E2[i];
+        const uint32_t dst = i - 2 * (lo + 1);
+        E[dst] = val;
+        G[dst] = (uint16_t)lo;
+      }
+    }
+    __syncthreads();
+    m -= 2 * c;
+    const uint64_t nf = 2ull * f + 1;
+    if (nf >= n) break;
+    const uint64_t nc = std::min<uint64_t>(2ull * c, n - nf);
+    f = (uint32_t)nf;
+    c = (uint32_t)nc;
+  }
+}
+
+// applyIndices (rebuilder.cu:217-245): gather every per-triangle array.
+__global__ void k_gather_geometry(const uint32_t* __restrict__ P, uint32_t N,
+                                  const float* __restrict__ iv, const float* __restrict__ in,
+                                  const float* __restrict__ iuv, const uint16_t* __restrict__ ish,
+                                  float* __restrict__ ov, float* __restrict__ on,
+                                  float* __restrict__ ouv, uint16_t* __restrict__ osh) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const size_t s = clampi(P[i], N);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    ov[(size_t)i * 9 + k] = iv[s * 9 + k];
+    on[(size_t)i * 9 + k] = in[s * 9 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ouv[(size_t)i * 6 + k] = iuv[s * 6 + k];
+  osh[i] = ish[s];
+}
+
+__device__ inline void slab_of(const float* v, int a, float& lo, float& hi) {
+  // intersect.cuh:25-33, 109-111 over the node's two triangles
+  const float* p = v;
+  const float* q = v + 9;
+  lo = fminf(fminf(fminf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]),
+             fminf(fminf(q[3 * a], q[3 * a + 1]), q[3 * a + 2]));
+  hi = fmaxf(fmaxf(fmaxf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]),
+             fmaxf(fmaxf(q[3 * a], q[3 * a + 1]), q[3 * a + 2]));
+}
+
+__global__ void k_node_records(const float* __restrict__ v, uint32_t n,
+                               NodeRec* __restrict__ nodes) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  NodeRec r;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const float* p = v + (size_t)(2 * k + t) * 9;  // points layout
+    const V3 v1 = mk(p[0], p[3], p[6]), v2 = mk(p[1], p[4], p[7]), v3 = mk(p[2], p[5], p[8]);
+    const V3 e1 = v2 - v1, e2 = v3 - v1;
+    r.tri[t][0] = v1.x;
+    r.tri[t][1] = v1.y;
+    r.tri[t][2] = v1.z;
+    r.tri[t][3] = e1.x;
+    r.tri[t][4] = e1.y;
+    r.tri[t][5] = e1.z;
+    r.tri[t][6] = e2.x;
+    r.tri[t][7] = e2.y;
+    r.tri[t][8] = e2.z;
+  }
+  const uint32_t d = depth_of(k);
+  const int ca = (int)((d + 1) % 3);
+  r.cslab[0] = r.cslab[2] = 0.0f;
+  r.cslab[1] = r.cslab[3] = 0.0f;
+  const uint64_t cl = 2ull * k + 1;
+  if (cl < n) slab_of(v + cl * 18, ca, r.cslab[0], r.cslab[1]);
+  if (cl + 1 < n) slab_of(v + (cl + 1) * 18, ca, r.cslab[2], r.cslab[3]);
+  slab_of(v + (size_t)k * 18, (int)(d % 3), r.pad[0], r.pad[1]);
+  nodes[k] = r;
+}
+
+// cudaGenerateTB (utility.cu:136-180).
+__global__ void k_generate_tb(const float* __restrict__ normals, uint32_t N,
+                              float* __restrict__ tangents, float* __restrict__ bitangents) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const V3 up = mk(0.0f, 1.0f, 0.0f), fwd = mk(0.0f, 0.0f, 1.0f);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const V3 nv = ld3(normals + (size_t)i * 9 + 3 * k);
+    V3 T = cross(nv, up);
+    if (length(T) < kEpsilon) T = cross(nv, fwd);
+    const V3 B = cross(nv, T);
+    float* tp = tangents + (size_t)i * 9 + 3 * k;
+    float* bp = bitangents + (size_t)i * 9 + 3 * k;
+    tp[0] = T.x;
+    tp[1] = T.y;
+    tp[2] = T.z;
+    bp[0] = B.x;
+    bp[1] = B.y;
+    bp[2] = B.z;
+  }
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+struct LevelPlan {
+  int level;
+  uint32_t m, first, nchildren, ngroups, ntiles;
+  size_t child_base, group_base, tile_base;
+};
+
+}  // namespace
+
+aur_status generate_tb(aur_context* ctx) {
+  if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "generate_tb: geometry not built");
+  const uint32_t N = ctx->count;
+  AUR_HIP(ctx, ctx->tangents.reserve((size_t)N * 9));
+  AUR_HIP(ctx, ctx->bitangents.reserve((size_t)N * 9));
+  k_generate_tb<<<blocks(N, 256), 256, 0, ctx->stream>>>(ctx->normals.p, N, ctx->tangents.p,
+                                                         ctx->bitangents.p);
+  AUR_LAUNCH_CHECK(ctx, "k_generate_tb");
+  ctx->has_tb = true;
+  return AUR_OK;
+}
+
+aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
+  const uint32_t N = ctx->count;
+  if (N < 2 || (N & 1u)) return ctx->fail(AUR_ERR_STATE, "rebuild: no even-sized geometry uploaded");
+  const uint32_t n = N / 2;
+  int L = 0;
+  for (uint32_t x = n; x; x >>= 1) ++L;  // log2i(n) + 1
+  hipStream_t s = ctx->stream;
+
+  // ---- host plan (closed forms only; no device data needed) ----
+  int level0 = 0;
+  if (N > kSubT) {
+    level0 = L - 1;
+    for (int l = 1; l < L; ++l) {
+      const uint64_t fp = (1ull << (l - 1)) - 1;  // first node of the parent level
+      const uint64_t maxg = 2 * desc_count(fp, fp + 1, n) - 2;
+      if (maxg <= kSubT) {
+        level0 = l;
+        break;
+      }
+    }
+  }
+  std::vector<TileDesc> tiles;
+  std::vector<ChildDesc> children;
+  std::vector<GroupDesc> groups;
+  std::vector<LevelPlan> plan;
+  size_t max_tiles = 1, max_children = 1;
+  for (int l = 0; l < level0; ++l) {
+    LevelPlan lp{};
+    lp.level = l;
+    lp.first = (uint32_t)((1ull << l) - 1);
+    const uint64_t last = std::min<uint64_t>((2ull << l) - 1, n);
+    lp.m = (uint32_t)(N - 2ull * lp.first);
+    lp.child_base = children.size();
+    lp.group_base = groups.size();
+    lp.tile_base = tiles.size();
+    uint64_t off = 0;
+    int64_t cur_parent = -1;
+    for (uint64_t h = lp.first; h < last; ++h) {
+      const uint64_t size = 2 * desc_count(h, h + 1, n);
+      const int64_t parent = l == 0 ? 0 : (int64_t)((h - 1) / 2);
+      if (parent != cur_parent) {
+        if (!groups.empty() && groups.size() > lp.group_base) groups.back().te = (uint32_t)(tiles.size() - lp.tile_base);
+        groups.push_back(GroupDesc{(uint32_t)off, 0, (uint32_t)(tiles.size() - lp.tile_base), 0});
+        cur_parent = parent;
+      }
+      GroupDesc& g = groups.back();
+      g.size += (uint32_t)size;
+      ChildDesc cd{(uint32_t)off, (uint32_t)size, (uint32_t)(tiles.size() - lp.tile_base), 0};
+      for (uint64_t st = 0; st < size; st += kTile)
+        tiles.push_back(TileDesc{(uint32_t)(off + st), (uint32_t)std::min<uint64_t>(kTile, size - st),
+                                 (uint32_t)(h - lp.first),
+                                 (uint32_t)(groups.size() - 1 - lp.group_base)});
+      cd.te = (uint32_t)(tiles.size() - lp.tile_base);
+      children.push_back(cd);
+      off += size;
+    }
+    groups.back().te = (uint32_t)(tiles.size() - lp.tile_base);
+    lp.nchildren = (uint32_t)(children.size() - lp.child_base);
+    lp.ngroups = (uint32_t)(groups.size() - lp.group_base);
+    lp.ntiles = (uint32_t)(tiles.size() - lp.tile_base);
+    max_tiles = std::max<size_t>(max_tiles, lp.ntiles);
+    max_children = std::max<size_t>(max_children, lp.nchildren);
+    plan.push_back(lp);
+  }
+  std::vector<SubGroup> sub;
+  if (level0 == 0) {
+    sub.push_back(SubGroup{0, N, 0, 1});
+  } else {
+    const uint64_t fp = (1ull << (level0 - 1)) - 1;
+    const uint64_t lastp = std::min<uint64_t>((1ull << level0) - 1, n);
+    const uint64_t fc = (1ull << level0) - 1;
+    for (uint64_t p = fp; p < lastp; ++p) {
+      const uint64_t c0 = 2 * p + 1;
+      if (c0 >= n) break;
+      const uint64_t c1 = std::min<uint64_t>(c0 + 2, n);
+      const uint64_t off = 2 * desc_count(fc, c0, n);
+      const uint64_t size = 2 * desc_count(c0, c1, n);
+      sub.push_back(SubGroup{(uint32_t)off, (uint32_t)size, (uint32_t)c0, (uint32_t)(c1 - c0)});
+    }
+  }
+
+  // ---- device buffers ----
+  DevBuf<float> bmin, bmax;
+  DevBuf<uint32_t> Wa, Wb, Ka, Kb, H, O, argpos;
+  DevBuf<uint2> partial;
+  DevBuf<TileDesc> dtiles;
+  DevBuf<ChildDesc> dchildren;
+  DevBuf<GroupDesc> dgroups;
+  DevBuf<SubGroup> dsub;
+  AUR_HIP(ctx, bmin.reserve((size_t)3 * N));
+  AUR_HIP(ctx, bmax.reserve((size_t)3 * N));
+  AUR_HIP(ctx, Wa.reserve(N));
+  AUR_HIP(ctx, Wb.reserve(N));
+  AUR_HIP(ctx, ctx->perm.reserve(N));
+  if (!plan.empty()) {
+    AUR_HIP(ctx, Ka.reserve(N));
+    AUR_HIP(ctx, Kb.reserve(N));
+    AUR_HIP(ctx, H.reserve(max_tiles * 256));
+    AUR_HIP(ctx, O.reserve(max_tiles * 256));
+    AUR_HIP(ctx, partial.reserve(max_tiles));
+    AUR_HIP(ctx, argpos.reserve(max_children));
+    AUR_HIP(ctx, dtiles.reserve(tiles.size()));
+    AUR_HIP(ctx, dchildren.reserve(children.size()));
+    AUR_HIP(ctx, dgroups.reserve(groups.size()));
+    AUR_HIP(ctx, hipMemcpy(dtiles.p, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+    AUR_HIP(ctx, hipMemcpy(dchildren.p, children.data(), children.size() * sizeof(ChildDesc),
+                           hipMemcpyHostToDevice));
+    AUR_HIP(ctx, hipMemcpy(dgroups.p, groups.data(), groups.size() * sizeof(GroupDesc),
+                           hipMemcpyHostToDevice));
+  }
+  AUR_HIP(ctx, dsub.reserve(sub.size()));
+  AUR_HIP(ctx, hipMemcpy(dsub.p, sub.data(), sub.size() * sizeof(SubGroup), hipMemcpyHostToDevice));
+
+  hipEvent_t e0, e1;
+  AUR_HIP(ctx, hipEventCreate(&e0));
+  AUR_HIP(ctx, hipEventCreate(&e1));
+  AUR_HIP(ctx, hipEventRecord(e0, s));
+
+  k_bounds<<<blocks(N, 256), 256, 0, s>>>(ctx->in_vertices.p, N, bmin.p, bmax.p);
+  k_iota<<<blocks(N, 256), 256, 0, s>>>(Wa.p, N);
+  k_fill0<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N);
+  uint32_t* Wcur = Wa.p;
+  uint32_t* Walt = Wb.p;
+  for (const LevelPlan& lp : plan) {
+    const int a = lp.level % 3;
+    const TileDesc* tl = dtiles.p + lp.tile_base;
+    const ChildDesc* ch = dchildren.p + lp.child_base;
+    const GroupDesc* gr = dgroups.p + lp.group_base;
+    k_gather_keys<<<blocks(lp.m, 256), 256, 0, s>>>(Wcur, lp.m, N, bmin.p + (size_t)a * N, Ka.p);
+    uint32_t *kin = Ka.p, *kout = Kb.p, *vin = Wcur, *vout = Walt;
+    for (int pass = 0; pass < 4; ++pass) {
+      const uint32_t shift = 8u * pass;
+      k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, shift, H.p);
+      k_radix_offsets<<<lp.ngroups, kThreads, 0, s>>>(gr, H.p, O.p);
+      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, O.p, lp.m, kout, vout);
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    // after 4 passes the sorted data is back in (Ka, Wcur)
+    k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Wcur, N, bmax.p + (size_t)a * N, partial.p);
+    k_argmax_combine<<<blocks(lp.nchildren, 256), 256, 0, s>>>(ch, lp.nchildren, partial.p, argpos.p);
+    k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Wcur, Walt, lp.first, lp.level, L,
+                                          bmin.p, N, ctx->perm.p);
+    std::swap(Wcur, Walt);
+  }
+  AUR_LAUNCH_CHECK(ctx, "nmh level kernels");
+  k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Wcur, bmin.p, bmax.p, N, n, L, level0,
+                                                      ctx->perm.p);
+  AUR_LAUNCH_CHECK(ctx, "k_subtree");
+
+  AUR_HIP(ctx, ctx->vertices.reserve((size_t)N * 9));
+  AUR_HIP(ctx, ctx->normals.reserve((size_t)N * 9));
+  AUR_HIP(ctx, ctx->texcoords.reserve((size_t)N * 6));
+  AUR_HIP(ctx, ctx->shader_ids.reserve(N));
+  AUR_HIP(ctx, ctx->nodes.reserve(n));
+  k_gather_geometry<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N, ctx->in_vertices.p,
+                                                   ctx->in_normals.p, ctx->in_texcoords.p,
+                                                   ctx->in_shader_ids.p, ctx->vertices.p,
+                                                   ctx->normals.p, ctx->texcoords.p,
+                                                   ctx->shader_ids.p);
+  k_node_records<<<blocks(n, 256), 256, 0, s>>>(ctx->vertices.p, n, ctx->nodes.p);
+  AUR_LAUNCH_CHECK(ctx, "gather/node records");
+  AUR_HIP(ctx, hipEventRecord(e1, s));
+  AUR_HIP(ctx, hipEventSynchronize(e1));
+  float ms = 0.0f;
+  AUR_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->levels = (uint32_t)L;
+  ctx->built = true;
+  ctx->has_tb = false;
+  if (stats) {
+    stats->count = N;
+    stats->levels = (uint32_t)L;
+    stats->build_ms = ms;
+  }
+  return AUR_OK;
+}
+
+}  // namespace aur

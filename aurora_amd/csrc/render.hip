@@ -1,0 +1,605 @@
+// Frame rendering on gfx950: replaces MonteCarloRaytracer::render
+// (src/render/monteCarloRaytracer.cpp:81-120) and the kernels it drives
+// (cudaGenerateRays, utility.cu:18-94; cudaRaytraceMonteCarlo,
+// monteCarloRaytracer.cu:20-124; cudaDrawPixels, utility.cu:199-229).
+//
+// AUR_MODE_COMPAT reproduces the reference's Monte Carlo ray tracer: one
+// thread per pixel sample walks the reference's ray stack (reflection and
+// refraction only, MAX_DEPTH 8), with the reference's direct-light
+// estimators (radiance.cuh:8-80, light.cuh:24-60) and shadow rays through
+// the shared traversal.  cuRAND is replaced by Philox keyed by
+// (seed, pixel, sample, draw index).
+//
+// AUR_MODE_PBR is a wavefront path loop: per sample pass, camera rays ->
+// [extend (closest hit) -> shade (emission, one next-event slot per light
+// sample, Russian roulette, cosine BSDF sample, compaction of surviving
+// paths into the next queue) -> shadow (any hit) -> resolve (adds the
+// unoccluded slots in light order)] x max_depth -> accumulate.  Float
+// operations follow the same order as the scalar restatement, so every
+// pixel replays exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "context.h"
+#include "traverse.h"
+
+namespace aur {
+namespace {
+
+constexpr int kBlock = 128;
+
+struct SceneDev {
+  const NodeRec* nodes;
+  uint32_t n_nodes;
+  const float* normals;
+  const uint16_t* shader_ids;
+  const aur_shader* shaders;
+  uint32_t num_shaders;
+  const aur_light* lights;
+  uint32_t num_lights;
+};
+
+__device__ inline const aur_shader& shader_of(const SceneDev& sc, uint32_t tri) {
+  uint32_t sid = sc.shader_ids[tri];
+  sid = sid > 0 ? sid - 1 : 0;  // getSafeID (src/util/array.h:18)
+  if (sid >= sc.num_shaders) sid = 0;
+  return sc.shaders[sid];
+}
+
+__device__ inline V3 interp9(const float* base, uint32_t tri, float u, float v) {
+  const float* p = base + (size_t)tri * 9;
+  return bclerp(mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), mk(p[6], p[7], p[8]), u, v);
+}
+
+// Wave-aggregated 64-bit counter increment.
+__device__ inline void count_add(unsigned long long* c, uint32_t v) {
+  uint32_t s = v;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63u) == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------- compat --
+
+struct CompatRng {
+  uint32_t seed, pixel, sample, k;
+  U4 cache;
+  __device__ float next() {
+    if ((k & 3u) == 0) cache = philox(U4{pixel, sample, 0x40000000u | (k >> 2), 0u}, seed, kKeyCompat);
+    const uint32_t w = (k & 3u) == 0 ? cache.x : (k & 3u) == 1 ? cache.y : (k & 3u) == 2 ? cache.z : cache.w;
+    ++k;
+    return u01(w);
+  }
+};
+
+__device__ inline V3 camera_dir_compat(const aur_camera& c, uint32_t w, uint32_t h, uint32_t x,
+                                       uint32_t y, uint32_t sampleID, uint32_t samples) {
+  float px, py;
+  if (samples == 1) {
+    px = float(x) / float(w) - 0.5f;
+    py = float(y) / float(h) - 0.5f;
+  } else {
+    const short swidth = (short)sqrtf((float)(unsigned short)samples);
+    const short sx = (short)(sampleID % swidth), sy = (short)(sampleID / swidth);
+    const float dx = 1.0f / float(w), dy = 1.0f / float(h);
+    const float sdx = dx / swidth, sdy = dy / swidth;
+    px = x * dx - 0.5f + (sx * sdx);
+    py = y * dy - 0.5f + (sy * sdy);
+  }
+  return normalize(px * ld3(c.right) * c.tanfov[0] + py * ld3(c.up) * c.tanfov[1] + ld3(c.forward));
+}
+
+struct CRay {
+  V3 pos, dir;
+  float t, weight, eta;
+};
+
+__device__ inline bool occluded(const SceneDev& sc, V3 o, V3 d, float tmax, float2* stk,
+                                uint32_t stride, uint32_t& nrays) {
+  ++nrays;
+  return trace_ray<true>(sc.nodes, sc.n_nodes, o, d, tmax, stk, stride).id >= 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, uint32_t w,
+                                                   uint32_t h, uint32_t spp, uint32_t sample,
+                                                   uint32_t seed, float4* __restrict__ pixels,
+                                                   unsigned long long* __restrict__ nrays_total) {
+  extern __shared__ float2 stk_mem[];
+  float2* stk = stk_mem + threadIdx.x;
+  const uint32_t stride = blockDim.x;
+  const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nrays = 0;
+  if (pix < w * h) {
+    const uint32_t x = pix % w, y = pix / w;
+    CompatRng rng{seed, pix, sample, 0u, U4{0, 0, 0, 0}};
+    const int kMaxDepth = 8;
+    CRay stack[kMaxDepth + 2];
+    int ns = 0;
+    stack[ns++] = CRay{ld3(cam.position), camera_dir_compat(cam, w, h, x, y, sample, spp), kInfinity,
+                       1.0f, 1.0f};
+    int depth = 0;
+    V3 color = mk(0.0f);
+    int32_t hpId = -1;
+    float hpU = 0.0f, hpV = 0.0f;
+    while (depth <= kMaxDepth && ns > 0) {
+      CRay ray = stack[--ns];
+      ++nrays;
+      const TraceResult hr = trace_ray<false>(sc.nodes, sc.n_nodes, ray.pos, ray.dir, ray.t, stk, stride);
+      if (hr.id < 0) continue;
+      ray.t = hr.t;
+      hpId = hr.id;
+      hpU = hr.u;
+      hpV = hr.v;
+      const V3 position = ray.pos + (ray.dir * ray.t);
+      const V3 wo = -ray.dir;
+      const aur_shader& sh = shader_of(sc, (uint32_t)hpId);
+      const V3 N = normalize(interp9(sc.normals, hpId, hpU, hpV));
+      const V3 color1 = ld3(sh.diffuse_color) * sh.diffuse;
+      V3 Li = ld3(sh.emission_color);
+      for (uint32_t i = 0; i < sc.num_lights; ++i) {
+        const aur_light& L = sc.lights[i];
+        const V3 CI = ld3(L.color) * L.intensity;
+        if (L.type == AUR_LIGHT_AMBIENT) Li = Li + (color1 * kInvPi) * (ld3(L.color) * L.intensity * L.ambient);
+        if (L.type != AUR_LIGHT_AREA) {
+          // estimateDirectRadianceDelta (radiance.cuh:13-28)
+          V3 wdir;
+          float wt;
+          V3 Lv;
+          if (L.type == AUR_LIGHT_AMBIENT) {
+            wdir = ld3(L.position) - position;
+            wt = length(wdir);
+            wdir = wdir / wt;
+            Lv = CI * (1.0f - L.ambient);
+          } else if (L.type == AUR_LIGHT_POINT) {
+            wdir = ld3(L.position) - position;
+            wt = length(wdir);
+            wdir = wdir / wt;
+            Lv = CI / (wt * wt);
+          } else {
+            wdir = -ld3(L.direction);
+            wt = kInfinity;
+            Lv = CI;
+          }
+          V3 add = mk(0.0f);
+          if (!is_zero(Lv)) {
+            const V3 f = color1 * kInvPi;
+            if (!is_zero(f) && !occluded(sc, position, wdir, wt, stk, stride, nrays))
+              add = f * Lv * fmaxf(0.0f, dot(wdir, N));
+          }
+          Li = Li + add;
+        } else {
+          // estimateDirectRadiance (radiance.cuh:30-80)
+          V3 acc = mk(0.0f);
+          for (int s = 0; s < L.samples; ++s) {
+            V3 Ls = mk(0.0f);
+            const float u1 = rng.next();
+            const float u2 = rng.next();
+            const V3 Pl = ld3(L.position) + ((ld3(L.e1) * u1) + (ld3(L.e2) * u2));
+            V3 wdir = Pl - position;
+            float wt = length(wdir);
+            wdir = wdir / wt;
+            const float lightPdf = 1.0f / fabsf(dot(ld3(L.direction), -wdir));
+            const V3 Lv = CI * fmaxf(0.0f, dot(ld3(L.direction), -wdir));
+            V3 wpos = position;
+            wpos.x += kEpsilon * wdir.x;
+            wpos.y += kEpsilon * wdir.y;
+            wpos.z += kEpsilon * wdir.z;
+            wt += kEpsilon;
+            if (lightPdf > 0.0f && !is_zero(Lv)) {
+              const V3 f = color1 * kInvPi;
+              if (!is_zero(f) && !occluded(sc, wpos, wdir, wt, stk, stride, nrays))
+                Ls = Ls + f * Lv * fmaxf(0.0f, dot(wdir, N));
+            }
+            acc = acc + Ls;
+          }
+          const float nsf = (float)L.samples;
+          Li = Li + mk(acc.x / nsf, acc.y / nsf, acc.z / nsf);
+        }
+      }
+      if (sh.reflectivity > 0.0f || sh.translucence > 0.0f) depth++;
+      if (depth <= kMaxDepth && sh.reflectivity > 0.0f) {
+        CRay nr{position, reflect(ray.dir, N), kInfinity, sh.reflectivity, ray.eta};
+        ray.weight *= (1.0f - sh.reflectivity);
+        stack[ns++] = nr;
+      }
+      if (depth <= kMaxDepth && sh.translucence > 0.0f) {
+        V3 rN = N;
+        float etai = ray.eta, etat = sh.refractive_index;
+        if (dot(rN, wo) < 0.0f) {
+          rN = -rN;
+          etai = sh.refractive_index;
+          etat = 1.0f;
+        }
+        V3 tdir;
+        if (refract(tdir, ray.dir, rN, etai, etat)) {
+          CRay nr{position, tdir, kInfinity, sh.translucence, etat};
+          ray.weight *= (1.0f - sh.translucence);
+          stack[ns++] = nr;
+        }
+      }
+      color = color + (Li * ray.weight);
+    }
+    if (hpId != -1) {
+      const float weight = 1.0f / (float)spp;
+      float4 px = pixels[pix];
+      px.x += color.x * weight;
+      px.y += color.y * weight;
+      px.z += color.z * weight;
+      px.w += weight;
+      pixels[pix] = px;
+    }
+  }
+  count_add(nrays_total, nrays);
+}
+
+// ------------------------------------------------------------------- PBR --
+
+__device__ inline U4 pbr_draw(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
+                              uint32_t slot) {
+  return philox(U4{pixel, sample, (bounce << 16) | slot, kTagPbr}, seed, kKeyPbr);
+}
+
+// sin/cos of (pi/4) q, |q| <= 1, Taylor polynomials in Horner form.
+__device__ inline void sincos_quarter(float q, float& s, float& c) {
+  const float phi = 0.785398163f * q;
+  const float z = phi * phi;
+  s = phi * (1.0f + z * (-0.166666667f + z * (0.00833333333f + z * (-0.000198412698f + z * 2.75573192e-6f))));
+  c = 1.0f + z * (-0.5f + z * (0.0416666667f + z * (-0.00138888889f + z * (2.48015873e-5f + z * -2.75573192e-7f))));
+}
+
+// Concentric map (Shirley & Chiu 1997) with the branch structure of
+// src/kernels/lib/sampling.cuh:51-73.
+__device__ inline void concentric_disk(float u1, float u2, float& dx, float& dy) {
+  const float x = 2.0f * u1 - 1.0f;
+  const float y = 2.0f * u2 - 1.0f;
+  float r, s, c;
+  if (x >= -y) {
+    if (x > y) {
+      r = x;
+      sincos_quarter(y / x, s, c);
+      dx = r * c;
+      dy = r * s;
+    } else {
+      r = y;
+      sincos_quarter(-x / y, s, c);
+      dx = r * -s;
+      dy = r * c;
+    }
+  } else {
+    if (x < y) {
+      r = -x;
+      sincos_quarter(y / x, s, c);
+      dx = r * -c;
+      dy = r * -s;
+    } else {
+      r = -y;
+      if (y != 0.0f) {
+        sincos_quarter(-x / y, s, c);
+        dx = r * s;
+        dy = r * -c;
+      } else {
+        dx = r;
+        dy = 0.0f;
+      }
+    }
+  }
+}
+
+// Orthonormal basis (Duff et al., JCGT 2017).
+__device__ inline void onb(V3 n, V3& t1, V3& t2) {
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  t1 = mk(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  t2 = mk(b, sign + n.y * n.y * a, -n.y);
+}
+
+struct PbrArgs {
+  uint32_t w, h, spp, max_depth, rr_start, seed, sample, nslots;
+  float4 *ray_o, *ray_d, *beta, *radiance;
+  aur_hit* hits;
+  float4 *sh_o, *sh_d, *sh_c;
+  int32_t* sh_occ;
+  uint32_t* counters;  // [0],[1] queue lengths
+  unsigned long long* stats;
+};
+
+__global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ queue) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    a.counters[0] = a.w * a.h;
+    a.counters[1] = 0;
+  }
+  if (p >= a.w * a.h) return;
+  const uint32_t x = p % a.w, y = p / a.w;
+  const U4 r = pbr_draw(a.seed, p, a.sample, 0, 0);
+  const float j0 = u01(r.x), j1 = u01(r.y);
+  uint32_t sw = 1;
+  while ((sw + 1) * (sw + 1) <= a.spp) ++sw;
+  float fx, fy;
+  if (sw * sw == a.spp) {
+    fx = ((float)(a.sample % sw) + j0) / (float)sw;
+    fy = ((float)(a.sample / sw) + j1) / (float)sw;
+  } else {
+    fx = j0;
+    fy = j1;
+  }
+  const float px = ((float)x + fx) / (float)a.w - 0.5f;
+  const float py = ((float)y + fy) / (float)a.h - 0.5f;
+  const V3 d = normalize(px * ld3(cam.right) * cam.tanfov[0] + py * ld3(cam.up) * cam.tanfov[1] +
+                         ld3(cam.forward));
+  a.ray_o[p] = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
+  a.ray_d[p] = make_float4(d.x, d.y, d.z, 0.0f);
+  a.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  a.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  queue[p] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, int cin,
+                                                       const uint32_t* __restrict__ queue) {
+  extern __shared__ float2 stk_mem[];
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_in = a.counters[cin];
+  // the other queue's length was last read by the previous bounce's kernels,
+  // which have completed; the shade kernel that follows appends to it
+  if (j == 0) a.counters[cin ^ 1] = 0;
+  const bool active = j < n_in;
+  if (active) {
+    const uint32_t p = queue[j];
+    const float4 o = a.ray_o[p], d = a.ray_d[p];
+    const TraceResult r = trace_ray<false>(sc.nodes, sc.n_nodes, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z),
+                                           kInfinity, stk_mem + threadIdx.x, blockDim.x);
+    a.hits[p] = aur_hit{r.id, r.t, r.u, r.v};
+  }
+  count_add(&a.stats[0], active ? 1u : 0u);
+}
+
+__global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
+                            const uint32_t* __restrict__ qin, uint32_t* __restrict__ qout) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_in = a.counters[cin];
+  if (j >= n_in) return;
+  const uint32_t p = qin[j];
+  const size_t base = (size_t)j * a.nslots;
+  const aur_hit hit = a.hits[p];
+  if (hit.id < 0) {
+    for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s].w = 0.0f;
+    return;
+  }
+  const float4 o4 = a.ray_o[p], d4 = a.ray_d[p], b4 = a.beta[p];
+  float4 L4 = a.radiance[p];
+  const V3 O = mk(o4.x, o4.y, o4.z), D = mk(d4.x, d4.y, d4.z);
+  V3 beta = mk(b4.x, b4.y, b4.z);
+  const V3 P = O + (D * hit.t);
+  const V3 wo = -D;
+  const aur_shader& sh = shader_of(sc, (uint32_t)hit.id);
+  V3 N = normalize(interp9(sc.normals, (uint32_t)hit.id, hit.u, hit.v));
+  if (dot(N, wo) < 0.0f) N = -N;
+  const V3 albedo = ld3(sh.diffuse_color) * sh.diffuse;
+  const V3 f = albedo * kInvPi;
+  V3 L = mk(L4.x, L4.y, L4.z) + beta * ld3(sh.emission_color);
+  a.radiance[p] = make_float4(L.x, L.y, L.z, bounce == 0 ? 1.0f : L4.w);
+  const V3 bf = beta * f;
+  uint32_t slot = 16, e = 0;
+  for (uint32_t li = 0; li < sc.num_lights; ++li) {
+    const aur_light& Lt = sc.lights[li];
+    const V3 CI = ld3(Lt.color) * Lt.intensity;
+    if (Lt.type == AUR_LIGHT_AREA) {
+      const int ns = Lt.samples > 0 ? Lt.samples : 1;
+      const float inv = 1.0f / (float)ns;
+      for (int k = 0; k < ns; ++k, ++e) {
+        const U4 r = pbr_draw(a.seed, p, a.sample, bounce, slot++);
+        const V3 Q = ld3(Lt.position) + ((ld3(Lt.e1) * u01(r.x)) + (ld3(Lt.e2) * u01(r.y)));
+        V3 wi = Q - P;
+        const float dist = length(wi);
+        wi = wi / dist;
+        const float cosL = dot(ld3(Lt.direction), -wi);
+        const float cosS = dot(N, wi);
+        float flag = 0.0f;
+        if (cosL > 0.0f && cosS > 0.0f) {
+          const float wgt = (cosS * cosL * Lt.area) / (dist * dist) * inv;
+          const V3 c = bf * (CI * wgt);
+          const V3 so = P + (kEpsilon * wi);
+          a.sh_o[base + e] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
+          a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+          flag = 1.0f;
+        }
+        a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+      }
+    } else if (Lt.type == AUR_LIGHT_POINT) {
+      V3 wi = ld3(Lt.position) - P;
+      const float dist = length(wi);
+      wi = wi / dist;
+      const float cosS = dot(N, wi);
+      float flag = 0.0f;
+      if (cosS > 0.0f) {
+        const V3 c = bf * ((CI / (dist * dist)) * cosS);
+        a.sh_o[base + e] = make_float4(P.x, P.y, P.z, dist);
+        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+        flag = 1.0f;
+      }
+      a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+      ++e;
+    } else if (Lt.type == AUR_LIGHT_DIRECTIONAL) {
+      const V3 wi = -ld3(Lt.direction);
+      const float cosS = dot(N, wi);
+      float flag = 0.0f;
+      if (cosS > 0.0f) {
+        const V3 c = bf * (CI * cosS);
+        a.sh_o[base + e] = make_float4(P.x, P.y, P.z, kInfinity);
+        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+        flag = 1.0f;
+      }
+      a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+      ++e;
+    } else {
+      const V3 c = bf * (CI * Lt.ambient);
+      a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+      a.sh_d[base + e] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);  // no shadow ray
+      ++e;
+    }
+  }
+  if (bounce + 1 >= a.max_depth) return;
+  const U4 r = pbr_draw(a.seed, p, a.sample, bounce, 1);
+  if (bounce >= a.rr_start) {
+    const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
+    if (u01(r.z) >= q) return;
+    beta = beta / q;
+  }
+  float dx, dy;
+  concentric_disk(u01(r.x), u01(r.y), dx, dy);
+  const float dz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
+  V3 t1, t2;
+  onb(N, t1, t2);
+  const V3 wi = t1 * dx + t2 * dy + N * dz;
+  beta = beta * albedo;
+  a.ray_o[p] = make_float4(P.x, P.y, P.z, 0.0f);
+  a.ray_d[p] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+  a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+  // compaction: one atomic per wave, lanes ranked by ballot
+  const uint64_t m = __ballot(true);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+  const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t basepos = 0;
+  if (lane == leader) basepos = atomicAdd(&a.counters[cin ^ 1], (uint32_t)__popcll(m));
+  basepos = __shfl(basepos, leader);
+  qout[basepos + rank] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, SceneDev sc, int cin) {
+  extern __shared__ float2 stk_mem[];
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)a.counters[cin] * a.nslots;
+  uint32_t traced = 0;
+  if (e < total) {
+    const float4 d = a.sh_d[e];
+    if (d.w == 1.0f) {
+      const float4 o = a.sh_o[e];
+      const TraceResult r = trace_ray<true>(sc.nodes, sc.n_nodes, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z),
+                                            o.w, stk_mem + threadIdx.x, blockDim.x);
+      a.sh_occ[e] = r.id >= 0 ? 1 : 0;
+      traced = 1;
+    }
+  }
+  count_add(&a.stats[1], traced);
+}
+
+__global__ void k_pbr_resolve(PbrArgs a, int cin, const uint32_t* __restrict__ qin) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.counters[cin]) return;
+  const uint32_t p = qin[j];
+  if (a.hits[p].id < 0) return;
+  const size_t base = (size_t)j * a.nslots;
+  float4 L4 = a.radiance[p];
+  V3 L = mk(L4.x, L4.y, L4.z);
+  for (uint32_t s = 0; s < a.nslots; ++s) {
+    const float flag = a.sh_d[base + s].w;
+    if (flag == 2.0f || (flag == 1.0f && a.sh_occ[base + s] == 0)) {
+      const float4 c = a.sh_c[base + s];
+      L = L + mk(c.x, c.y, c.z);
+    }
+  }
+  a.radiance[p] = make_float4(L.x, L.y, L.z, L4.w);
+}
+
+__global__ void k_pbr_accumulate(PbrArgs a, float4* __restrict__ pixels) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.w * a.h) return;
+  const float4 L = a.radiance[p];
+  if (L.w != 0.0f) {
+    const float weight = 1.0f / (float)a.spp;
+    float4 px = pixels[p];
+    px.x += L.x * weight;
+    px.y += L.y * weight;
+    px.z += L.z * weight;
+    px.w += weight;
+    pixels[p] = px;
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+inline uint32_t nblocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o,
+                        uint32_t s0, uint32_t s1) {
+  FrameState& fr = ctx->frame;
+  if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "render: geometry not built");
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "render: no frame");
+  if (ctx->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "render: no shaders uploaded");
+  if (o->spp == 0 || s1 > o->spp || s0 > s1) return ctx->fail(AUR_ERR_INVALID_ARG, "render: bad sample range");
+  if (o->mode == AUR_MODE_PBR && o->max_depth == 0) return ctx->fail(AUR_ERR_INVALID_ARG, "render: max_depth 0");
+  if (o->mode != AUR_MODE_COMPAT && o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "render: mode");
+  hipStream_t s = ctx->stream;
+  const uint32_t w = fr.width, h = fr.height, npix = w * h;
+  SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
+              ctx->num_shaders, ctx->lights.p, ctx->num_lights};
+  const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
+  AUR_HIP(ctx, fr.counters.reserve(8));
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
+  k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
+  if (o->mode == AUR_MODE_COMPAT) {
+    for (uint32_t smp = s0; smp < s1; ++smp) {
+      k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, o->spp, smp, o->seed,
+                                                          fr.pixels.p, stats);
+    }
+    AUR_LAUNCH_CHECK(ctx, "k_compat");
+  } else {
+    uint32_t nslots = 0;
+    {
+      // slot count per path: area lights use `samples` slots, others one
+      std::vector<aur_light> hl(ctx->num_lights);
+      if (ctx->num_lights)
+        AUR_HIP(ctx, hipMemcpy(hl.data(), ctx->lights.p, sizeof(aur_light) * ctx->num_lights,
+                               hipMemcpyDeviceToHost));
+      for (const aur_light& l : hl) nslots += l.type == AUR_LIGHT_AREA ? (uint32_t)std::max<int>(1, l.samples) : 1u;
+    }
+    const size_t nsh = std::max<size_t>(1, (size_t)npix * nslots);
+    AUR_HIP(ctx, fr.ray_o.reserve(npix));
+    AUR_HIP(ctx, fr.ray_d.reserve(npix));
+    AUR_HIP(ctx, fr.beta.reserve(npix));
+    AUR_HIP(ctx, fr.radiance.reserve(npix));
+    AUR_HIP(ctx, fr.hits.reserve(npix));
+    AUR_HIP(ctx, fr.queue[0].reserve(npix));
+    AUR_HIP(ctx, fr.queue[1].reserve(npix));
+    AUR_HIP(ctx, fr.sh_o.reserve(nsh));
+    AUR_HIP(ctx, fr.sh_d.reserve(nsh));
+    AUR_HIP(ctx, fr.sh_c.reserve(nsh));
+    AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
+    PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots,
+              fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
+              fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats};
+    for (uint32_t smp = s0; smp < s1; ++smp) {
+      a.sample = smp;
+      k_pbr_camera<<<nblocks(npix, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
+      int cin = 0;
+      for (uint32_t b = 0; b < o->max_depth; ++b) {
+        uint32_t* qin = fr.queue[cin].p;
+        uint32_t* qout = fr.queue[cin ^ 1].p;
+        k_pbr_extend<<<nblocks(npix, kBlock), kBlock, lds, s>>>(a, sc, cin, qin);
+        k_pbr_shade<<<nblocks(npix, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
+        if (nslots) {
+          k_pbr_shadow<<<nblocks((uint64_t)npix * nslots, kBlock), kBlock, lds, s>>>(a, sc, cin);
+          k_pbr_resolve<<<nblocks(npix, 256), 256, 0, s>>>(a, cin, qin);
+        }
+        cin ^= 1;
+      }
+      k_pbr_accumulate<<<nblocks(npix, 256), 256, 0, s>>>(a, fr.pixels.p);
+    }
+    AUR_LAUNCH_CHECK(ctx, "pbr kernels");
+  }
+  unsigned long long hs[2] = {0, 0};
+  AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+  AUR_HIP(ctx, hipStreamSynchronize(s));
+  fr.rays_traced = hs[0] + hs[1];
+  return AUR_OK;
+}
+
+}  // namespace aur

@@ -1,0 +1,160 @@
+"""Host-side mirror of the reference's renderer layer over the C ABI.
+
+Same names, argument meaning and status behaviour as
+  Renderer            src/core/renderer.h:21-42
+  MonteCarloRaytracer src/render/monteCarloRaytracer.h:14-40, .cpp:23-127
+  Scene::update       src/core/scene.cpp:398-449 (mesh/shader/light upload,
+                      rebuild, generateTB)
+with Maya's MStatus replaced by the Status values below and MDagPath by an
+aur_camera record (src/util/camera.h:12-36).
+"""
+from __future__ import annotations
+
+import dataclasses
+import enum
+
+import numpy as np
+
+from . import native, scenes
+
+
+class Status(enum.IntEnum):
+    kSuccess = 0
+    kFailure = 1
+    kInsufficientMemory = 2
+
+
+def _status_of(err: native.AuroraError) -> Status:
+    return Status.kInsufficientMemory if err.status == 1 else Status.kFailure
+
+
+@dataclasses.dataclass
+class Rect:
+    """src/util/math.h:283-297 (inclusive pixel bounds)."""
+
+    left: int
+    right: int
+    bottom: int
+    top: int
+
+
+@dataclasses.dataclass
+class Dim:
+    """src/util/math.h:300-313; depth = samples per pixel."""
+
+    width: int
+    height: int
+    depth: int = 1
+
+
+class Scene:
+    """Scene upload and NMH rebuild (Scene::update / updateMeshes,
+    src/core/scene.cpp:85-240, 398-449)."""
+
+    def __init__(self, ctx: native.Context):
+        self.ctx = ctx
+        self.build_stats = None
+        self.source: scenes.Scene | None = None
+
+    def update(self, scene: scenes.Scene, full: bool = True) -> Status:
+        try:
+            if full:
+                self.ctx.upload_geometry(scene.vertices, scene.normals, scene.texcoords,
+                                         scene.shader_ids)
+            self.ctx.upload_shaders(scene.shaders)
+            self.ctx.upload_lights(scene.lights)
+            if full:
+                self.build_stats = self.ctx.rebuild_nmh()  # rebuild + generateTB
+            self.source = scene
+        except native.AuroraError as e:
+            return _status_of(e)
+        return Status.kSuccess
+
+
+class Renderer:
+    """Abstract renderer (src/core/renderer.h:21-42)."""
+
+    def createFrame(self, width, height, samples, scene, camera) -> Status:  # noqa: N802
+        raise NotImplementedError
+
+    def destroyFrame(self) -> Status:  # noqa: N802
+        raise NotImplementedError
+
+    def setRegion(self, region: Rect) -> Status:  # noqa: N802
+        raise NotImplementedError
+
+    def update(self) -> Status:
+        return Status.kSuccess
+
+    def render(self, engine=None, ipr: bool = False) -> Status:
+        raise NotImplementedError
+
+    def framebuffer(self):
+        raise NotImplementedError
+
+
+class MonteCarloRaytracer(Renderer):
+    """src/render/monteCarloRaytracer.cpp:23-127 over aur_frame_*.
+
+    mode selects the reference-compatible estimator (native.MODE_COMPAT) or
+    the path loop (native.MODE_PBR); seed keys the counter-based RNG that
+    replaces the tick-count-seeded cuRAND states (monteCarloRaytracer.cpp:54).
+    """
+
+    def __init__(self, ctx: native.Context, mode: int = native.MODE_COMPAT, max_depth: int = 4,
+                 rr_start: int = 3, seed: int = 1):
+        self.ctx = ctx
+        self.mode, self.max_depth, self.rr_start, self.seed = mode, max_depth, rr_start, seed
+        self.m_size: Dim | None = None
+        self.m_region: Rect | None = None
+        self.m_camera = None
+        self.m_scene = None
+
+    def createFrame(self, width, height, samples, scene, camera) -> Status:  # noqa: N802
+        try:
+            self.ctx.frame_create(width, height, samples)
+        except native.AuroraError as e:
+            return _status_of(e)
+        self.m_scene = scene
+        self.m_camera = camera
+        self.m_region = Rect(0, width - 1, 0, height - 1)
+        self.m_size = Dim(width, height, samples)
+        return Status.kSuccess
+
+    def destroyFrame(self) -> Status:  # noqa: N802
+        try:
+            self.ctx.frame_destroy()
+        except native.AuroraError as e:
+            return _status_of(e)
+        self.m_size = None
+        return Status.kSuccess
+
+    def setRegion(self, region: Rect) -> Status:  # noqa: N802
+        # the reference stores the region but its ray generation always maps
+        # the full frame (utility.cu:79-94); only full-frame regions exist here
+        if self.m_size is None:
+            return Status.kFailure
+        if (region.left, region.bottom) != (0, 0) or region.right != self.m_size.width - 1 or \
+                region.top != self.m_size.height - 1:
+            return Status.kFailure
+        self.m_region = region
+        return Status.kSuccess
+
+    def render(self, engine=None, ipr: bool = False) -> Status:
+        if self.m_size is None:
+            return Status.kFailure
+        opts = native.Context.opts(self.mode, self.m_size.depth, self.max_depth, self.rr_start,
+                                   self.seed)
+        try:
+            self.ctx.frame_render(self.m_camera, opts)
+        except native.AuroraError as e:
+            return _status_of(e)
+        return Status.kSuccess
+
+    def framebuffer(self) -> np.ndarray | None:
+        """float RGBA (height, width, 4), row 0 = bottom; None on failure."""
+        try:
+            px = self.ctx.frame_read()
+        except native.AuroraError:
+            return None
+        return px.reshape(self.m_size.height, self.m_size.width, 4)

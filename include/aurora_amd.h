@@ -1,0 +1,180 @@
+/* aurora_amd: MI355X (gfx950) render core of the Aurora renderer, C ABI.
+ *
+ * This is the drop-in boundary below the reference's renderer layer.  The
+ * reference calls C++ free functions declared in src/kernels/kernels.h:25-64
+ * from its Renderer/Scene/Geometry classes; each entry point below names the
+ * reference interface it replaces.  Conventions:
+ *   - every call returns aur_status (0 = AUR_OK); nothing throws across the
+ *     ABI; aur_last_error() describes the last failure of a context;
+ *   - a context owns one HIP device stream and all device memory it allocates;
+ *     a context is used by one thread at a time, distinct contexts are
+ *     independent;
+ *   - "host" pointers are ordinary CPU memory and the call is synchronous;
+ *     "device" pointers are HIP device memory and the work is queued on the
+ *     context's stream (aur_stream / aur_synchronize).
+ *
+ * Geometry layout (src/data/geometry.h:13-55): per triangle, vertices in the
+ * "points" layout x1 x2 x3 y1 y2 y3 z1 z2 z3 (9 floats), normals in the
+ * "values" layout v1xyz v2xyz v3xyz (9 floats), 6 texture coordinates and a
+ * uint16 shader id (1-based, 0 = default, src/util/array.h:12-18).
+ */
+#ifndef AURORA_AMD_H
+#define AURORA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AUR_ABI_VERSION 1u
+
+typedef int32_t aur_status;
+enum {
+  AUR_OK = 0,
+  AUR_ERR_OOM = 1,
+  AUR_ERR_INVALID_ARG = 2,
+  AUR_ERR_DEVICE = 3,
+  AUR_ERR_CANCELLED = 4,
+  AUR_ERR_UNSUPPORTED = 5,
+  AUR_ERR_STATE = 6
+};
+
+typedef struct aur_context aur_context;
+
+/* Shader record, field for field the reference's Shader (80 B,
+ * src/util/shader.h:15-43; attribute mapping src/core/scene.cpp:242-291). */
+typedef struct aur_shader {
+  int32_t type; /* 0 Lambert, 1 Phong */
+  float diffuse_color[3];
+  float emission_color[3];
+  float specular_color[3];
+  float reflection_color[3];
+  float diffuse, emission, exponent, reflectivity, translucence, refractive_index;
+  uint32_t texture; /* 1-based; 0 = none */
+} aur_shader;
+
+/* Light record, field for field the reference's Light (80 B,
+ * src/util/light.h:14-51; built at src/core/scene.cpp:328-396). */
+enum { AUR_LIGHT_AMBIENT = 0, AUR_LIGHT_POINT = 1, AUR_LIGHT_DIRECTIONAL = 2, AUR_LIGHT_AREA = 3 };
+typedef struct aur_light {
+  int32_t type;
+  float intensity, ambient, area;
+  int16_t samples, pad;
+  float position[3], color[3], direction[3], e1[3], e2[3];
+} aur_light;
+
+/* Pinhole camera as built by the reference host code (src/util/camera.h:12-36). */
+typedef struct aur_camera {
+  float position[3], forward[3], right[3], up[3], tanfov[2], aspect;
+} aur_camera;
+
+/* Ray record for the trace entry points: origin, tmax (the reference's
+ * Ray::t, src/util/ray.h:44-48), direction.  The slab range always starts at
+ * Epsilon = 1e-7 as in src/kernels/lib/intersect.cuh:40,100. */
+typedef struct aur_ray {
+  float org[3];
+  float tmax;
+  float dir[3];
+  float reserved;
+} aur_ray;
+
+/* Hit record: tree-order triangle index (-1 on miss), t (tmax on miss) and
+ * barycentrics; the reference's HitPoint fields triangleID/u/v
+ * (src/util/hitpoint.h:10-24) plus the accepted ray.t. */
+typedef struct aur_hit {
+  int32_t id;
+  float t, u, v;
+} aur_hit;
+
+enum { AUR_MODE_COMPAT = 0, AUR_MODE_PBR = 1 };
+typedef struct aur_render_opts {
+  uint32_t abi_version; /* AUR_ABI_VERSION */
+  uint32_t mode;        /* AUR_MODE_COMPAT: the reference's MC ray tracer; AUR_MODE_PBR: path loop */
+  uint32_t spp;         /* samples per pixel (the reference's Dim::depth) */
+  uint32_t max_depth;   /* PBR: bounces per path */
+  uint32_t rr_start;    /* PBR: first bounce that applies Russian roulette */
+  uint32_t seed;        /* counter-based RNG key */
+} aur_render_opts;
+
+typedef struct aur_build_stats {
+  uint32_t count;   /* triangles */
+  uint32_t levels;  /* log2i(count/2)+1 */
+  float build_ms;   /* device time of the build (bounds .. gather) */
+} aur_build_stats;
+
+/* ---- context (replaces Engine's device setup, src/core/engine.cpp:33-63) ---- */
+aur_status aur_create(int32_t device, aur_context** out);
+void aur_destroy(aur_context* ctx);
+const char* aur_last_error(const aur_context* ctx);
+uint32_t aur_abi_version(void);
+void* aur_stream(aur_context* ctx); /* hipStream_t of the context */
+aur_status aur_synchronize(aur_context* ctx);
+
+/* ---- scene upload (replaces Geometry::resize/copyToDevice and
+ * Scene::updateShaders/updateLights uploads: src/data/geometry.cpp:88-151,
+ * src/core/scene.cpp:242-396).  Host arrays; the caller keeps ownership.
+ * count must be even (Geometry::resize pads to even, geometry.cpp:90); -0.0
+ * is stored as +0.0; NaN/Inf vertices are rejected. ---- */
+aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* vertices,
+                               const float* normals, const float* texcoords,
+                               const uint16_t* shader_ids);
+aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader* shaders);
+aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* lights);
+
+/* ---- NMH build (replaces cudaRebuildNMH(Geometry&), src/kernels/rebuilder.cu:311-450,
+ * declared at src/kernels/kernels.h:313).  Reorders the uploaded geometry into
+ * tree order on the device and prepares the traversal records. ---- */
+aur_status aur_rebuild_nmh(aur_context* ctx, aur_build_stats* stats);
+
+/* ---- tangent frames (replaces cudaGenerateTB, src/kernels/utility.cu:169-180,
+ * kernels.h:303).  Runs on the tree-order geometry. ---- */
+aur_status aur_generate_tb(aur_context* ctx);
+
+/* Read back (host) for parity checks.  perm[i] = original index of the
+ * triangle at tree position i.  Any array pointer may be NULL. */
+aur_status aur_read_permutation(aur_context* ctx, uint32_t* perm);
+aur_status aur_read_geometry(aur_context* ctx, float* vertices, float* normals, float* texcoords,
+                             uint16_t* shader_ids, float* tangents, float* bitangents);
+uint32_t aur_geometry_count(const aur_context* ctx);
+
+/* ---- traversal (the device functions intersect()/intersectAny(),
+ * src/kernels/lib/intersect.cuh:36-152, as standalone launches).  Device
+ * pointers; queued on the context stream. ---- */
+aur_status aur_trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits);
+aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* hit_ids);
+/* Closest hit plus the reference loop's traversal counters summed over the
+ * rays: popped = states popped with index < count, passed = states whose slab
+ * test passed.  Synchronous.  Used for the algorithmic-bytes model
+ * 32 + 16 + 24*popped + 48*passed bytes per ray. */
+aur_status aur_trace_closest_stats(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
+                                   uint64_t* popped, uint64_t* passed);
+
+/* ---- frame (replaces MonteCarloRaytracer::createFrame/destroyFrame/
+ * setRegion/render/framebuffer, src/render/monteCarloRaytracer.cpp:23-127,
+ * with the ray generation / trace / draw kernels of kernels.h:296-321). ---- */
+aur_status aur_frame_create(aur_context* ctx, uint32_t width, uint32_t height, uint32_t spp);
+aur_status aur_frame_destroy(aur_context* ctx);
+/* Clears the accumulation buffer and renders opts->spp samples. */
+aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
+                            const aur_render_opts* opts);
+/* Renders samples [sample_begin, sample_end) of opts->spp into the existing
+ * accumulation (progressive rendering and resume; the counter RNG makes a
+ * split render equal to an unsplit one). */
+aur_status aur_frame_render_samples(aur_context* ctx, const aur_camera* camera,
+                                    const aur_render_opts* opts, uint32_t sample_begin,
+                                    uint32_t sample_end);
+aur_status aur_frame_clear(aur_context* ctx);
+/* Host float RGBA, width*height*4 floats, row 0 = bottom (framebuffer(),
+ * src/render/monteCarloRaytracer.cpp:122-127). */
+aur_status aur_frame_read(aur_context* ctx, float* rgba);
+/* Device pointer of the float4 accumulation buffer. */
+aur_status aur_frame_device_buffer(aur_context* ctx, float** rgba);
+/* Number of ray segments (camera + extension + shadow) traced by the last
+ * aur_frame_render / aur_frame_render_samples call. */
+uint64_t aur_frame_rays_traced(const aur_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AURORA_AMD_H */

@@ -1,0 +1,170 @@
+"""ctypes bindings to the CPU oracle (oracle/liboracle.so).
+
+The oracle is the checker: tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = ctypes.CDLL(ORACLE_SO)
+        P = ctypes.c_void_p
+        u32, i32, u64 = ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
+        L.orc_nmh_build.argtypes = [u32, P, P]
+        L.orc_nmh_build.restype = i32
+        L.orc_nmh_levels.argtypes = [u32]
+        L.orc_nmh_levels.restype = u32
+        L.orc_calc_partition.argtypes = [u32, P, P]
+        L.orc_trace.argtypes = [u32, P, u64, P, i32, P, P, i32]
+        L.orc_trace.restype = i32
+        L.orc_ray_triangle.argtypes = [P, P, P]
+        L.orc_ray_triangle.restype = i32
+        L.orc_generate_tb.argtypes = [u32, P, P, P]
+        L.orc_camera_ray.argtypes = [P, u32, u32, u32, u32, u32, u32, P]
+        L.orc_render.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, u32, P, P, u32, P, i32]
+        L.orc_render.restype = i32
+        L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
+        L.orc_philox.argtypes = [P, u32, u32, P]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def calc_partition(size: int) -> tuple[int, int]:
+    L = np.zeros(1, np.uint32)
+    R = np.zeros(1, np.uint32)
+    lib().orc_calc_partition(size, _p(L), _p(R))
+    return int(L[0]), int(R[0])
+
+
+def nmh_levels(count: int) -> int:
+    return int(lib().orc_nmh_levels(count))
+
+
+def nmh_build(vertices: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, dtype=np.float32)
+    n = v.shape[0]
+    out = np.empty(n, dtype=np.uint32)
+    rc = lib().orc_nmh_build(n, _p(v), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_nmh_build failed: {rc}")
+    return out
+
+
+def trace(tree_vertices: np.ndarray, rays: np.ndarray, any_hit: bool = False, threads: int = 0,
+          counters: bool = False):
+    v = np.ascontiguousarray(tree_vertices, dtype=np.float32)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    n = r.shape[0]
+    hits = np.zeros((n, 4), dtype=np.uint32)
+    cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
+    th = threads or os.cpu_count() or 1
+    rc = lib().orc_trace(v.shape[0], _p(v), n, _p(r), int(any_hit), _p(hits),
+                         _p(cnt) if counters else None, th)
+    if rc != 0:
+        raise RuntimeError(f"orc_trace failed: {rc}")
+    ids = hits[:, 0].view(np.int32).copy()
+    tuv = hits[:, 1:4].view(np.float32).copy()
+    return (ids, tuv, cnt) if counters else (ids, tuv)
+
+
+def ray_triangle(ray8: np.ndarray, tri9: np.ndarray):
+    r = np.ascontiguousarray(ray8, dtype=np.float32)
+    t = np.ascontiguousarray(tri9, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    ok = lib().orc_ray_triangle(_p(r), _p(t), _p(out))
+    return bool(ok), out
+
+
+def generate_tb(normals: np.ndarray):
+    n = np.ascontiguousarray(normals, dtype=np.float32)
+    T = np.empty_like(n)
+    B = np.empty_like(n)
+    lib().orc_generate_tb(n.shape[0], _p(n), _p(T), _p(B))
+    return T, B
+
+
+def camera_ray(camera: np.ndarray, w: int, h: int, x: int, y: int, sample: int, samples: int):
+    cam = np.ascontiguousarray(camera)
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_camera_ray(cam.ctypes.data_as(ctypes.c_void_p), w, h, x, y, sample, samples, _p(out))
+    return out
+
+
+def render(tree, scene, mode: int, spp: int, max_depth: int, rr_start: int, seed: int,
+           pixels: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+    """tree: dict with tree-order arrays (vertices, normals, tangents,
+    bitangents, shader_ids).  Returns float32 (npix, 4) accumulations."""
+    opts = np.array([mode, spp, max_depth, rr_start, seed], dtype=np.uint32)
+    w, h = scene.width, scene.height
+    px = None if pixels is None else np.ascontiguousarray(pixels, dtype=np.uint32)
+    npix = w * h if px is None else px.shape[0]
+    out = np.zeros((npix, 4), dtype=np.float32)
+    v = np.ascontiguousarray(tree["vertices"], np.float32)
+    nrm = np.ascontiguousarray(tree["normals"], np.float32)
+    T = np.ascontiguousarray(tree["tangents"], np.float32)
+    B = np.ascontiguousarray(tree["bitangents"], np.float32)
+    ids = np.ascontiguousarray(tree["shader_ids"], np.uint16)
+    sh = np.ascontiguousarray(scene.shaders)
+    li = np.ascontiguousarray(scene.lights)
+    cam = np.ascontiguousarray(scene.camera)
+    th = threads or os.cpu_count() or 1
+    rc = lib().orc_render(v.shape[0], _p(v), _p(nrm), _p(T), _p(B), _p(ids), sh.shape[0],
+                          sh.ctypes.data_as(ctypes.c_void_p), li.shape[0],
+                          li.ctypes.data_as(ctypes.c_void_p) if li.shape[0] else None,
+                          cam.ctypes.data_as(ctypes.c_void_p), w, h, _p(opts),
+                          _p(px) if px is not None else None, npix, _p(out), th)
+    if rc != 0:
+        raise RuntimeError(f"orc_render failed: {rc}")
+    return out
+
+
+def tree_arrays(scene, perm: np.ndarray) -> dict:
+    """Gather a scene's triangle arrays into tree order (applyIndices,
+    src/kernels/rebuilder.cu:217-245) and add the tangent frames."""
+    nrm = scene.normals[perm]
+    T, B = generate_tb(nrm)
+    return {
+        "vertices": scene.vertices[perm],
+        "normals": nrm,
+        "texcoords": scene.texcoords[perm],
+        "shader_ids": scene.shader_ids[perm],
+        "tangents": T,
+        "bitangents": B,
+    }
+
+
+def concentric_disk(u1: float, u2: float):
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_concentric_disk(u1, u2, _p(out))
+    return out
+
+
+def philox(ctr, k0: int, k1: int) -> np.ndarray:
+    c = np.asarray(ctr, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(_p(c), k0, k1, _p(out))
+    return out
