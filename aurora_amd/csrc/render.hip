@@ -260,11 +260,15 @@ __device__ inline void concentric_disk(float u1, float u2, float& dx, float& dy)
       sincos_quarter(y / x, s, c);
       dx = r * c;
       dy = r * s;
-    } else {
+    } else if (y != 0.0f) {
       r = y;
       sincos_quarter(-x / y, s, c);
       dx = r * -s;
       dy = r * c;
+    } else {
+      // centre of the square: the reference's 0/0 would give NaN
+      dx = 0.0f;
+      dy = 0.0f;
     }
   } else {
     if (x < y) {

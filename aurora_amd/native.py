@@ -53,7 +53,7 @@ _lib = None
 
 def header_symbols() -> list[str]:
     """Every function declared by include/aurora_amd.h."""
-    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", " ", open(HEADER).read(), flags=re.S)
     return sorted(set(re.findall(r"\b(aur_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -225,7 +225,7 @@ class Context:
 
     def trace(self, rays: np.ndarray, any_hit: bool = False):
         """Host convenience: copies rays (n, 8) float32 to the device with torch,
-        traces, and returns (ids int32, tuv float32 (n,3)) like the oracle."""
+        traces, and returns (ids int32 (n,), t/u/v float32 (n, 3))."""
         import torch
 
         r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).to("cuda")

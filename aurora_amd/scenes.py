@@ -475,6 +475,44 @@ def random_soup(seed: int, n: int, box: float = 10.0, grid: float | None = None)
 
 
 # --------------------------------------------------------------------------
+# analytic scenes (closed-form expectations exist for these)
+
+
+def plane_under_light(albedo: float = 0.5, radiance: float = 10.0, light_height: float = 2.0,
+                      light_size: float = 1.0, width: int = 32, height: int = 32) -> Scene:
+    """A 10 m x 10 m Lambert floor (y = 0, facing up) under one downward
+    facing square area light: nothing occludes, so a floor point's direct
+    light has closed forms (polygon irradiance; compat estimator quadrature)."""
+    parts = [_quad((-5.0, 0.0, -5.0), (0.0, 0.0, 10.0), (10.0, 0.0, 0.0))]
+    ids = [np.full(2, 1)]
+    h = 0.5 * light_size
+    light = area_light((-h, light_height, -h), (light_size, 0.0, 0.0), (0.0, 0.0, light_size),
+                       (0.0, -1.0, 0.0), (radiance, radiance, radiance))
+    cam = make_camera((0.0, 3.0, -4.0), (0.0, -3.0, 4.0), (0.6, 0.6 * height / width))
+    return _finish(parts, ids, [_lambert((albedo, albedo, albedo))], [light], cam, width=width,
+                   height=height, spp=16, max_depth=1, name="plane-under-light")
+
+
+def furnace(albedo: float = 0.5, emission: float = 1.0, subdivisions: int = 2, width: int = 32,
+            height: int = 32, max_depth: int = 5) -> Scene:
+    """Closed, inward-facing emissive Lambert sphere with the camera at its
+    centre and no lights: every path hits at every bounce, so the PBR path
+    loop's expected radiance is emission * sum_{k<depth} albedo^k."""
+    V, F = icosphere(subdivisions)
+    F = F[:, [0, 2, 1]]  # inward winding
+    P, _ = _sphere_tris(V, F, (0.0, 0.0, 0.0), 1.0)
+    # flat (face) normals: a sampled direction never dips below its own
+    # triangle, so no path can leave the closed sphere
+    n = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    Nm = np.broadcast_to(n[:, None, :], P.shape).copy()
+    cam = make_camera((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (1.0, 1.0 * height / width))
+    sh = _lambert((albedo, albedo, albedo), (emission, emission, emission))
+    return _finish([(P, Nm)], [np.full(P.shape[0], 1)], [sh], [], cam, width=width, height=height,
+                   spp=4, max_depth=max_depth, name="furnace")
+
+
+# --------------------------------------------------------------------------
 # ray sets: records of 8 float32 (ox oy oz tmax dx dy dz 0)
 
 

@@ -346,11 +346,15 @@ inline void concentric_disk(float u1, float u2, float& dx, float& dy) {
       sincos_quarter(y / x, s, c);
       dx = r * c;
       dy = r * s;
-    } else {
+    } else if (y != 0.0f) {
       r = y;
       sincos_quarter(-x / y, s, c);  // angle pi/2 + (pi/4) q
       dx = r * -s;
       dy = r * c;
+    } else {
+      // x = y = 0: the reference divides 0/0 here and returns NaN
+      dx = 0.0f;
+      dy = 0.0f;
     }
   } else {
     if (x < y) {

@@ -1,0 +1,245 @@
+"""Pins the oracle's sampling, shading frames, direct lighting and path loop.
+
+* Philox4x32-10 against the published Random123 known-answer vectors.
+* Concentric disk map against the textbook formula (real sin/cos, float64)
+  and its equal-area property (chi-square over equal-area cells).
+* cudaGenerateTB's frames (src/kernels/utility.cu:136-167): orthogonality and
+  the fallback axis.
+* Camera rays (src/kernels/utility.cu:18-77, src/util/camera.h:18-27) against
+  the closed form.
+* Compat direct light (src/kernels/lib/radiance.cuh:30-80, F2: no 1/r^2, no
+  area, no pdf) z-tested per pixel against an FP64 quadrature of the
+  estimator's expectation.
+* PBR next-event estimation z-tested against Lambert's closed-form polygon
+  irradiance (which is itself checked against FP64 quadrature).
+* PBR bounce throughput and Russian roulette: furnace (closed emissive
+  sphere), expected radiance emission * sum albedo^k.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from aurora_amd import scenes
+
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expected", KAT)
+def test_philox_known_answers(oracle, ctr, key, expected):
+    assert [int(x) for x in oracle.philox(ctr, *key)] == list(expected)
+
+
+def _concentric64(u1, u2):
+    # src/kernels/lib/sampling.cuh:51-73 in float64 with real cos/sin
+    x, y = 2 * u1 - 1, 2 * u2 - 1
+    if x >= -y:
+        if x > y:
+            r, th = x, math.pi / 4 * (y / x)
+        else:
+            r, th = y, math.pi / 4 * (2 - x / y)
+    else:
+        if x < y:
+            r, th = -x, math.pi / 4 * (4 + y / x)
+        else:
+            r, th = -y, (math.pi / 4 * (6 - x / y)) if y != 0 else 0.0
+    return r * math.cos(th), r * math.sin(th)
+
+
+def test_concentric_disk_formula(oracle):
+    rng = np.random.default_rng(0)
+    for u1, u2 in rng.random((3000, 2)):
+        d = oracle.concentric_disk(np.float32(u1), np.float32(u2))
+        ref = _concentric64(float(np.float32(u1)), float(np.float32(u2)))
+        assert abs(d[0] - ref[0]) < 2e-6 and abs(d[1] - ref[1]) < 2e-6
+    for u1, u2 in [(0.5, 0.5), (0.0, 0.0), (1.0, 0.5), (0.5, 1.0), (0.0, 1.0)]:
+        d = oracle.concentric_disk(u1, u2)
+        assert d[0] * d[0] + d[1] * d[1] <= 1.0 + 1e-6
+
+
+def test_concentric_disk_equal_area(oracle):
+    # 8 angular sectors x 4 equal-area rings: uniform counts (chi-square)
+    rng = np.random.default_rng(1)
+    n = 32000
+    counts = np.zeros((8, 4))
+    for u1, u2 in rng.random((n, 2)):
+        dx, dy = oracle.concentric_disk(np.float32(u1), np.float32(u2))
+        r2 = dx * dx + dy * dy
+        a = (math.atan2(dy, dx) + 2 * math.pi) % (2 * math.pi)
+        counts[min(7, int(a / (2 * math.pi) * 8)), min(3, int(r2 * 4))] += 1
+    exp = n / 32
+    chi2 = ((counts - exp) ** 2 / exp).sum()
+    assert chi2 < 70  # 31 dof: p ~ 1e-4
+
+
+def test_tangent_frames(oracle):
+    rng = np.random.default_rng(2)
+    nrm = rng.normal(size=(500, 3, 3))
+    nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+    nrm[0, 0] = (0.0, 1.0, 0.0)  # parallel to up: falls back to n x forward
+    nrm[0, 1] = (0.0, -1.0, 0.0)
+    n9 = nrm.reshape(-1, 9).astype(np.float32)
+    T, B = oracle.generate_tb(n9)
+    T, B = T.reshape(-1, 3, 3).astype(np.float64), B.reshape(-1, 3, 3).astype(np.float64)
+    assert np.abs(np.sum(T * nrm, axis=2)).max() < 1e-5
+    assert np.abs(np.sum(B * nrm, axis=2)).max() < 1e-5
+    assert np.abs(np.sum(B * T, axis=2)).max() < 1e-5
+    np.testing.assert_allclose(T[0, 0], np.cross(nrm[0, 0], (0, 0, 1)), atol=1e-6)
+    # B = n x T
+    np.testing.assert_allclose(B, np.cross(nrm, T), atol=1e-5)
+
+
+def test_camera_rays_closed_form(oracle):
+    s = scenes.cornell(width=64, height=32)
+    cam = s.camera
+    f64 = lambda k: np.asarray(cam[k], dtype=np.float64)  # noqa: E731
+    for (x, y, i, spp) in [(32, 16, 0, 1), (0, 0, 0, 1), (5, 7, 3, 16), (63, 31, 15, 16), (10, 3, 8, 9)]:
+        d = oracle.camera_ray(cam, 64, 32, x, y, i, spp)
+        if spp == 1:
+            px, py = x / 64 - 0.5, y / 32 - 0.5
+        else:
+            sw = int(math.isqrt(spp))
+            px = x / 64 - 0.5 + (i % sw) * (1 / 64) / sw
+            py = y / 32 - 0.5 + (i // sw) * (1 / 32) / sw
+        ref = px * f64("right") * cam["tanfov"][0] + py * f64("up") * cam["tanfov"][1] + f64("forward")
+        ref /= np.linalg.norm(ref)
+        np.testing.assert_allclose(d, ref, atol=2e-6)
+
+
+# --- direct light: FP64 expectations ----------------------------------------
+
+
+def _floor_hits(s, pxy):
+    """Float64 camera rays through continuous pixel coordinates pxy (k, 2) in
+    [-0.5, 0.5); hit points on the y = 0 floor."""
+    cam = s.camera
+    r = np.asarray(cam["right"], np.float64)
+    u = np.asarray(cam["up"], np.float64)
+    fw = np.asarray(cam["forward"], np.float64)
+    d = pxy[:, :1] * r * cam["tanfov"][0] + pxy[:, 1:2] * u * cam["tanfov"][1] + fw
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.asarray(cam["position"], np.float64)
+    t = -o[1] / d[:, 1]
+    return o + t[:, None] * d
+
+
+def _light_grid(L, n=64):
+    g = (np.arange(n) + 0.5) / n
+    u1, u2 = np.meshgrid(g, g)
+    P = np.asarray(L["position"], np.float64) + u1.reshape(-1, 1) * np.asarray(L["e1"], np.float64) + \
+        u2.reshape(-1, 1) * np.asarray(L["e2"], np.float64)
+    return P
+
+
+def _integrand(X, L, kind):
+    """Per light point values of one sample's contribution (gray channel)."""
+    P = _light_grid(L)
+    w = P[None, :, :] - X[:, None, :]
+    dist = np.linalg.norm(w, axis=2)
+    w = w / dist[..., None]
+    n_l = np.asarray(L["direction"], np.float64)
+    cosL = -(w @ n_l)
+    cosS = w[..., 1]
+    CI = float(L["color"][0]) * float(L["intensity"])
+    if kind == "compat":
+        g = CI * np.maximum(0, cosL) * np.maximum(0, cosS)
+    else:
+        g = np.where((cosL > 0) & (cosS > 0), CI * cosS * cosL * float(L["area"]) / dist ** 2, 0.0)
+    return g  # (points, light samples)
+
+
+def polygon_form_factor(X, verts):
+    """Lambert's formula: integral over the polygon of cos cos' / r^2 dA for a
+    point X with normal +y: 1/2 sum_i angle_i * (n . unit(r_i x r_i+1))."""
+    total = np.zeros(X.shape[0])
+    k = len(verts)
+    for i in range(k):
+        a = verts[i] - X
+        b = verts[(i + 1) % k] - X
+        an = a / np.linalg.norm(a, axis=1, keepdims=True)
+        bn = b / np.linalg.norm(b, axis=1, keepdims=True)
+        ang = np.arccos(np.clip(np.sum(an * bn, axis=1), -1, 1))
+        c = np.cross(a, b)
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        total += ang * c[:, 1]
+    return np.abs(0.5 * total)
+
+
+def test_polygon_formula_matches_quadrature():
+    s = scenes.plane_under_light()
+    L = s.lights[0]
+    X = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 2.0], [-3.0, 0.0, 1.0]])
+    corner, e1, e2 = (np.asarray(L[k], np.float64) for k in ("position", "e1", "e2"))
+    ff = polygon_form_factor(X, [corner, corner + e1, corner + e1 + e2, corner + e2])
+    g = _integrand(X, L, "pbr").mean(axis=1) / (float(L["color"][0]) * float(L["intensity"]))
+    np.testing.assert_allclose(g, ff, rtol=2e-3)
+
+
+def _zscores(s, obs, pix, spp, kind, albedo):
+    w, h = s.width, s.height
+    L = s.lights[0]
+    f = albedo / math.pi
+    sw = int(math.isqrt(spp))
+    zs = []
+    for q, p in enumerate(pix):
+        x, y = p % w, p // w
+        if kind == "compat":
+            sub = np.array([[x / w - 0.5 + (i % sw) * (1 / w) / sw, y / h - 0.5 + (i // sw) * (1 / h) / sw]
+                            for i in range(spp)])
+        else:
+            g = (np.arange(8) + 0.5) / 8
+            gx, gy = np.meshgrid(g, g)
+            sub = np.stack([(x + gx.ravel()) / w - 0.5, (y + gy.ravel()) / h - 0.5], 1)
+        X = _floor_hits(s, sub)
+        vals = f * _integrand(X, L, kind)  # (sub-positions, light points)
+        if kind == "compat":
+            mean = vals.mean()
+            var = vals.var(axis=1).sum() / spp ** 2
+        else:
+            mean = vals.mean()
+            var = vals.var() / spp
+        zs.append((obs[q] - mean) / math.sqrt(var))
+    return np.array(zs)
+
+
+@pytest.mark.parametrize("kind,mode", [("compat", 0), ("pbr", 1)])
+def test_direct_light_z_test(oracle, kind, mode):
+    albedo = 0.5
+    s = scenes.plane_under_light(albedo=albedo)
+    perm = oracle.nmh_build(s.vertices)
+    tree = oracle.tree_arrays(s, perm)
+    rng = np.random.default_rng(3)
+    pix = rng.choice(s.width * s.height, 64, replace=False).astype(np.uint32)
+    spp = 64
+    img = oracle.render(tree, s, mode, spp, 1, 3, 11, pixels=pix)
+    assert np.all(img[:, 3] == 1.0)  # every pixel sees the floor
+    z = _zscores(s, img[:, 0].astype(np.float64), pix, spp, kind, albedo)
+    assert np.abs(z).max() < 5.0, z
+    assert abs(z.mean()) < 0.7, z.mean()
+
+
+def test_furnace_exact_without_roulette(oracle):
+    s = scenes.furnace(albedo=0.5, emission=1.0, max_depth=5)
+    perm = oracle.nmh_build(s.vertices)
+    img = oracle.render(oracle.tree_arrays(s, perm), s, 1, 4, 5, 99, 5)
+    expected = sum(0.5 ** k for k in range(5))
+    close = np.abs(img[:, 0] - expected) <= 1e-5 * expected
+    assert close.mean() >= 0.995, close.mean()
+
+
+def test_furnace_roulette_unbiased(oracle):
+    s = scenes.furnace(albedo=0.6, emission=1.0, max_depth=6, width=48, height=48)
+    perm = oracle.nmh_build(s.vertices)
+    img = oracle.render(oracle.tree_arrays(s, perm), s, 1, 16, 6, 1, 21)
+    expected = sum(0.6 ** k for k in range(6))
+    v = img[:, 0].astype(np.float64)
+    se = v.std() / math.sqrt(v.size)
+    assert abs(v.mean() - expected) < 5 * se + 1e-6, (v.mean(), expected, se)
+    assert v.std() > 0  # roulette does randomise the estimate
