@@ -38,16 +38,17 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def parse_traffic(kernel_prefix: str):
-    """HBM bytes per launch from committed rocprofv3 --pmc summaries, if any."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+def parse_traffic(kernel: str, summary: str):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) from the
+    committed rocprofv3 summary's 'traffic_bytes_per_launch <kernel> <B>' line."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
-    try:
-        data = json.load(open(path))
-        return data.get(kernel_prefix, {}).get("bytes_per_launch")
-    except Exception:
-        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
 
 
 def cpu_baseline(tree_vertices: np.ndarray, rays: np.ndarray, budget_s: float = 15.0):
@@ -80,7 +81,21 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "launches": 5}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               "(the render core is single-GPU; see DESIGN.md for how it would shard)",
@@ -145,7 +160,7 @@ def main() -> None:
         cpu = cpu_baseline(scene.vertices[perm], rays_np)
         log(f"cpu baseline {cpu['value']} Mrays/s on {cpu['cores']} threads")
 
-    traffic = parse_traffic("k_trace")
+    traffic = parse_traffic("k_trace", "k_trace_incoherent_1M.md")
     out = {
         "metric": "Mrays/s @1M tri (incoherent closest hit)",
         "value": round(mrays, 2),
