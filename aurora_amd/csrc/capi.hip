@@ -40,11 +40,32 @@ aur_status aur_create(int32_t device, aur_context** out) {
   aur_context* ctx = new (std::nothrow) aur_context();
   if (!ctx) return AUR_ERR_OOM;
   ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->num_cus = (uint32_t)cus;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return AUR_ERR_DEVICE;
   }
   *out = ctx;
+  return AUR_OK;
+}
+
+aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* p) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (!p) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: null");
+  if (p->block != 64 && p->block != 128 && p->block != 256)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: block must be 64, 128 or 256");
+  if (p->persistent && p->refill != 1 && p->refill != 8 && p->refill != 16 && p->refill != 32)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: refill must be 1, 8, 16 or 32");
+  if (p->waves_per_cu > 64) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: waves_per_cu > 64");
+  ctx->trace_params = *p;
+  return AUR_OK;
+}
+
+aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* p) {
+  if (!ctx || !p) return AUR_ERR_INVALID_ARG;
+  *p = ctx->trace_params;
   return AUR_OK;
 }
 

@@ -63,6 +63,10 @@ struct aur_context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string last_error;
+  uint32_t num_cus = 256;
+  // persistent waves, 256-thread blocks, refill at 16 idle lanes, occupancy
+  // limited residency: fastest shape measured at 1M triangles / 2^24 rays
+  aur_trace_params trace_params{1, 256, 16, 0};
 
   // uploaded geometry (input order)
   uint32_t count = 0;

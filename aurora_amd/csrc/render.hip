@@ -29,6 +29,7 @@ namespace aur {
 namespace {
 
 constexpr int kBlock = 128;
+constexpr int kRefill = 16;  // persistent traversal: refill once 16 lanes idle
 
 struct SceneDev {
   const NodeRec* nodes;
@@ -340,23 +341,28 @@ __global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ q
   queue[p] = p;
 }
 
+// Closest hit for every queued path: persistent waves over the queue.
 __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, int cin,
                                                        const uint32_t* __restrict__ queue) {
   extern __shared__ float2 stk_mem[];
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_in = a.counters[cin];
-  // the other queue's length was last read by the previous bounce's kernels,
-  // which have completed; the shade kernel that follows appends to it
-  if (j == 0) a.counters[cin ^ 1] = 0;
-  const bool active = j < n_in;
-  if (active) {
-    const uint32_t p = queue[j];
-    const float4 o = a.ray_o[p], d = a.ray_d[p];
-    const TraceResult r = trace_ray<false>(sc.nodes, sc.n_nodes, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z),
-                                           kInfinity, stk_mem + threadIdx.x, blockDim.x);
-    a.hits[p] = aur_hit{r.id, r.t, r.u, r.v};
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the other queue's length was last read by the previous bounce's
+    // kernels, which have completed; the shade kernel that follows appends
+    a.counters[cin ^ 1] = 0;
+    atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
-  count_add(&a.stats[0], active ? 1u : 0u);
+  persistent_trace<false, kRefill>(
+      sc.nodes, sc.n_nodes, n_in, stk_mem + threadIdx.x, blockDim.x,
+      [&](uint64_t j, V3& o, V3& d, float& tmax) {
+        const uint32_t p = queue[j];
+        const float4 o4 = a.ray_o[p], d4 = a.ray_d[p];
+        o = mk(o4.x, o4.y, o4.z);
+        d = mk(d4.x, d4.y, d4.z);
+        tmax = kInfinity;
+        return true;
+      },
+      [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
 }
 
 __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
@@ -472,21 +478,24 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
   qout[basepos + rank] = p;
 }
 
+// Any hit for every next-event slot that needs a shadow ray.
 __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, SceneDev sc, int cin) {
   extern __shared__ float2 stk_mem[];
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t total = (uint64_t)a.counters[cin] * a.nslots;
   uint32_t traced = 0;
-  if (e < total) {
-    const float4 d = a.sh_d[e];
-    if (d.w == 1.0f) {
-      const float4 o = a.sh_o[e];
-      const TraceResult r = trace_ray<true>(sc.nodes, sc.n_nodes, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z),
-                                            o.w, stk_mem + threadIdx.x, blockDim.x);
-      a.sh_occ[e] = r.id >= 0 ? 1 : 0;
-      traced = 1;
-    }
-  }
+  persistent_trace<true, kRefill>(
+      sc.nodes, sc.n_nodes, total, stk_mem + threadIdx.x, blockDim.x,
+      [&](uint64_t e, V3& o, V3& d, float& tmax) {
+        const float4 d4 = a.sh_d[e];
+        if (d4.w != 1.0f) return false;
+        const float4 o4 = a.sh_o[e];
+        o = mk(o4.x, o4.y, o4.z);
+        d = mk(d4.x, d4.y, d4.z);
+        tmax = o4.w;
+        ++traced;
+        return true;
+      },
+      [&](uint64_t e, const Trav& s) { a.sh_occ[e] = s.id >= 0 ? 1 : 0; });
   count_add(&a.stats[1], traced);
 }
 
@@ -529,6 +538,18 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
 }
 
 inline uint32_t nblocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+// Resident grid of a persistent kernel: occupancy-limited blocks on every CU,
+// but no more waves than max_items/64.
+uint32_t resident_grid(aur_context* ctx, const void* fn, size_t lds, uint64_t max_items) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  uint64_t blocks = (uint64_t)per_cu * ctx->num_cus;
+  const uint64_t need = (max_items + kBlock - 1) / kBlock;
+  if (blocks > need) blocks = need;
+  return (uint32_t)(blocks ? blocks : 1);
+}
 
 }  // namespace
 
@@ -580,6 +601,8 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
     PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots,
               fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
               fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats};
+    const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npix);
+    const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, (uint64_t)npix * nslots);
     for (uint32_t smp = s0; smp < s1; ++smp) {
       a.sample = smp;
       k_pbr_camera<<<nblocks(npix, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
@@ -587,10 +610,10 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
       for (uint32_t b = 0; b < o->max_depth; ++b) {
         uint32_t* qin = fr.queue[cin].p;
         uint32_t* qout = fr.queue[cin ^ 1].p;
-        k_pbr_extend<<<nblocks(npix, kBlock), kBlock, lds, s>>>(a, sc, cin, qin);
+        k_pbr_extend<<<grid_extend, kBlock, lds, s>>>(a, sc, cin, qin);
         k_pbr_shade<<<nblocks(npix, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
         if (nslots) {
-          k_pbr_shadow<<<nblocks((uint64_t)npix * nslots, kBlock), kBlock, lds, s>>>(a, sc, cin);
+          k_pbr_shadow<<<grid_shadow, kBlock, lds, s>>>(a, sc, cin);
           k_pbr_resolve<<<nblocks(npix, 256), 256, 0, s>>>(a, cin, qin);
         }
         cin ^= 1;

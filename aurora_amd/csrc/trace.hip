@@ -1,7 +1,16 @@
-// Standalone traversal launches: one lane per ray, closest hit or any hit
-// (src/kernels/lib/intersect.cuh:36-152).  Each lane keeps its deferred
-// ranges in an LDS column of `levels` float2 entries.
+// Standalone traversal launches (src/kernels/lib/intersect.cuh:36-152).
+//
+// k_trace_pt: persistent waves (persistent_trace in traverse.h): each wave
+// owns a contiguous range of rays and refills idle lanes from it, so lanes
+// whose rays finish early do not idle until the slowest ray of the wave ends.
+// Each lane keeps its deferred ranges in an LDS column of `levels` float2
+// entries.
+//
+// k_trace_flat: one lane per ray, no refill (reference schedule; used for
+// comparison and by the instrumented counting launch).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "context.h"
 #include "traverse.h"
@@ -9,34 +18,60 @@
 namespace aur {
 namespace {
 
-constexpr int kTraceBlock = 128;
+constexpr int kMaxBlock = 256;
+
+__device__ __forceinline__ void store_result(uint64_t i, const Trav& s, aur_hit* __restrict__ hits,
+                                             int32_t* __restrict__ ids) {
+  if (ids)
+    ids[i] = s.id;
+  else
+    reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(s.id), s.t, s.u, s.v);
+}
+
+template <bool kAny, int kRefill>
+__global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restrict__ nodes,
+                                                        uint32_t n_nodes,
+                                                        const aur_ray* __restrict__ rays, uint64_t n,
+                                                        aur_hit* __restrict__ hits,
+                                                        int32_t* __restrict__ ids) {
+  extern __shared__ float2 stk_mem[];
+  persistent_trace<kAny, kRefill>(
+      nodes, n_nodes, n, stk_mem + threadIdx.x, blockDim.x,
+      [&](uint64_t i, V3& o, V3& d, float& tmax) {
+        const float4* r = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = r[0], b = r[1];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmax = a.w;
+        return true;
+      },
+      [&](uint64_t i, const Trav& st) { store_result(i, st, hits, ids); });
+}
 
 template <bool kAny>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const NodeRec* __restrict__ nodes,
-                                                       uint32_t n_nodes,
-                                                       const aur_ray* __restrict__ rays, uint64_t n,
-                                                       aur_hit* __restrict__ hits,
-                                                       int32_t* __restrict__ ids) {
+__global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restrict__ nodes,
+                                                          uint32_t n_nodes,
+                                                          const aur_ray* __restrict__ rays,
+                                                          uint64_t n, aur_hit* __restrict__ hits,
+                                                          int32_t* __restrict__ ids) {
   extern __shared__ float2 stk_mem[];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* r = reinterpret_cast<const float4*>(rays + i);
   const float4 a = r[0], b = r[1];
-  const TraceResult res = trace_ray<kAny>(nodes, n_nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
-                                          stk_mem + threadIdx.x, blockDim.x);
-  if (kAny) {
-    ids[i] = res.id;
-  } else {
-    reinterpret_cast<float4*>(hits)[i] =
-        make_float4(__int_as_float(res.id), res.t, res.u, res.v);
+  Trav s;
+  if (trav_init<false>(s, nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
+    while (!trav_step<kAny, false>(s, nodes, n_nodes, stk_mem + threadIdx.x, blockDim.x, nullptr)) {
+    }
   }
+  store_result(i, s, hits, ids);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_count(const NodeRec* __restrict__ nodes,
-                                                             uint32_t n_nodes,
-                                                             const aur_ray* __restrict__ rays,
-                                                             uint64_t n, aur_hit* __restrict__ hits,
-                                                             unsigned long long* __restrict__ sums) {
+__global__ __launch_bounds__(kMaxBlock) void k_trace_count(const NodeRec* __restrict__ nodes,
+                                                           uint32_t n_nodes,
+                                                           const aur_ray* __restrict__ rays,
+                                                           uint64_t n, aur_hit* __restrict__ hits,
+                                                           unsigned long long* __restrict__ sums) {
   extern __shared__ float2 stk_mem[];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   TraceCount c;
@@ -63,6 +98,48 @@ __global__ void k_zero2(unsigned long long* p) {
   if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
 }
 
+template <bool kAny>
+aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
+                        int32_t* ids) {
+  const aur_trace_params& tp = ctx->trace_params;
+  const uint32_t block = tp.block;
+  const size_t lds = (size_t)ctx->levels * block * sizeof(float2);
+  if (!tp.persistent) {
+    const uint64_t nb = (n + block - 1) / block;
+    if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
+    k_trace_flat<kAny><<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays,
+                                                                  n, hits, ids);
+    AUR_LAUNCH_CHECK(ctx, "k_trace_flat");
+    return AUR_OK;
+  }
+  // resident waves: occupancy-limited blocks per CU times CUs (or the request)
+  int per_cu = 0;
+  const void* fn = nullptr;
+  switch (tp.refill) {
+    case 1: fn = (const void*)k_trace_pt<kAny, 1>; break;
+    case 8: fn = (const void*)k_trace_pt<kAny, 8>; break;
+    case 16: fn = (const void*)k_trace_pt<kAny, 16>; break;
+    case 32: fn = (const void*)k_trace_pt<kAny, 32>; break;
+    default: return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be 1, 8, 16 or 32");
+  }
+  AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds));
+  if (per_cu < 1) per_cu = 1;
+  uint64_t blocks_total = (uint64_t)per_cu * ctx->num_cus;
+  if (tp.waves_per_cu) blocks_total = std::max<uint64_t>(1, (uint64_t)tp.waves_per_cu * ctx->num_cus * 64 / block);
+  const uint64_t max_waves = (n + 63) / 64;  // more waves than rays/64 would idle
+  uint64_t nwaves = blocks_total * block / 64;
+  if (nwaves > max_waves) nwaves = max_waves;
+  const uint32_t grid = (uint32_t)((nwaves * 64 + block - 1) / block);
+  switch (tp.refill) {
+    case 1: k_trace_pt<kAny, 1><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
+    case 8: k_trace_pt<kAny, 8><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
+    case 16: k_trace_pt<kAny, 16><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
+    default: k_trace_pt<kAny, 32><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
+  }
+  AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
+  return AUR_OK;
+}
+
 }  // namespace
 
 aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
@@ -73,11 +150,12 @@ aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays
   AUR_HIP(ctx, sums.reserve(2));
   k_zero2<<<1, 64, 0, ctx->stream>>>(sums.p);
   if (n) {
-    const uint64_t nb = (n + kTraceBlock - 1) / kTraceBlock;
+    const uint32_t block = 128;
+    const uint64_t nb = (n + block - 1) / block;
     if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
-    const size_t lds = (size_t)ctx->levels * kTraceBlock * sizeof(float2);
-    k_trace_count<<<(uint32_t)nb, kTraceBlock, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays,
-                                                                    n, hits, sums.p);
+    const size_t lds = (size_t)ctx->levels * block * sizeof(float2);
+    k_trace_count<<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n,
+                                                              hits, sums.p);
     AUR_LAUNCH_CHECK(ctx, "k_trace_count");
   }
   unsigned long long h[2] = {0, 0};
@@ -92,26 +170,14 @@ aur_status trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_
   if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "trace: geometry not built");
   if (n == 0) return AUR_OK;
   if (!rays || !hits) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: null buffer");
-  const uint64_t nb = (n + kTraceBlock - 1) / kTraceBlock;
-  if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
-  const size_t lds = (size_t)ctx->levels * kTraceBlock * sizeof(float2);
-  k_trace<false><<<(uint32_t)nb, kTraceBlock, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays,
-                                                                   n, hits, nullptr);
-  AUR_LAUNCH_CHECK(ctx, "k_trace<closest>");
-  return AUR_OK;
+  return launch_trace<false>(ctx, n, rays, hits, nullptr);
 }
 
 aur_status trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* ids) {
   if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "trace: geometry not built");
   if (n == 0) return AUR_OK;
   if (!rays || !ids) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: null buffer");
-  const uint64_t nb = (n + kTraceBlock - 1) / kTraceBlock;
-  if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
-  const size_t lds = (size_t)ctx->levels * kTraceBlock * sizeof(float2);
-  k_trace<true><<<(uint32_t)nb, kTraceBlock, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays,
-                                                                  n, nullptr, ids);
-  AUR_LAUNCH_CHECK(ctx, "k_trace<any>");
-  return AUR_OK;
+  return launch_trace<true>(ctx, n, rays, nullptr, ids);
 }
 
 }  // namespace aur

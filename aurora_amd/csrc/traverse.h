@@ -6,7 +6,7 @@
 // (one-sided Moller-Trumbore, ray.cuh:22-47, strict t < ray.t so the first
 // hit wins ties), then pushes R and L (L popped first; L/R swapped when the
 // ray direction on the children's axis is negative).  The same tests run here
-// in the same order, with three changes that do not alter any result:
+// in the same order, with changes that do not alter any result:
 //   * a node's children slabs are stored in the node record, so both
 //     children are tested when their parent is visited; the t-dependent part
 //     of the test (range.x <= ray.t) is still applied when a child is taken,
@@ -15,18 +15,40 @@
 //   * a deferred child is the sibling of the current path's node at its
 //     depth, so the per-lane stack holds only its range (8 bytes in LDS,
 //     indexed by depth) plus one trail bit per depth; the child's heap index
-//     is recomputed from the last visited node when it is resumed.
-// Division stays the correctly rounded IEEE quotient, as the reference's
-// culling and t values depend on it.
+//     is recomputed from the last visited node when it is resumed;
+//   * the slab quotient (s - p) / d uses the ray's correctly rounded
+//     reciprocal y = RN(1/d) and one residual correction,
+//         q0 = RN(n y),  e = n - d q0 (exact, FMA),  q = RN(q0 + e y),
+//     which equals RN(n / d) (Markstein's theorem) while no step under- or
+//     overflows: used only for 2^-60 <= |d| <= 16 and n = 0 or
+//     2^-96 <= |n| <= 2^60, true division otherwise.
+// Triangle tests keep the correctly rounded 1/det of the reference.
+//
+// The traversal is a small state machine (trav_init / trav_step, one node
+// visit per step) so that persistent kernels can refill finished lanes.
 #pragma once
 
 #include "common.h"
 
 namespace aur {
 
-__device__ __forceinline__ float2 narrow(float2 r, float slo, float shi, float p, float d) {
-  const float d0 = (slo - p) / d;
-  const float d1 = (shi - p) / d;
+__device__ __forceinline__ bool fast_num(float n) {
+  const float a = fabsf(n);
+  return n == 0.0f || (a >= 0x1p-96f && a <= 0x1p60f);
+}
+
+__device__ __forceinline__ float2 narrow(float2 r, float slo, float shi, float p, float d, float y,
+                                         bool fast_d) {
+  const float n0 = slo - p, n1 = shi - p;
+  float d0, d1;
+  if (fast_d && fast_num(n0) && fast_num(n1)) {
+    const float q0 = n0 * y, q1 = n1 * y;
+    d0 = __builtin_fmaf(__builtin_fmaf(-d, q0, n0), y, q0);
+    d1 = __builtin_fmaf(__builtin_fmaf(-d, q1, n1), y, q1);
+  } else {
+    d0 = n0 / d;
+    d1 = n1 / d;
+  }
   return make_float2(fmaxf(fminf(d0, d1), r.x), fminf(fmaxf(d0, d1), r.y));
 }
 
@@ -58,87 +80,193 @@ struct TraceCount {
   uint32_t popped = 0, passed = 0;
 };
 
+struct Trav {
+  V3 o, d, y;      // origin, direction, RN(1/direction)
+  uint32_t fast;   // bit a: axis a uses the reciprocal quotient
+  float t, u, v;   // current closest hit (t = ray.t)
+  int32_t id;
+  uint32_t k, depth, trail;
+  uint32_t axis;   // depth % 3
+  float2 r;        // range of node k after its slab test
+};
+
+__device__ __forceinline__ float compf(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// Sets up a ray; returns false when the root's slab culls it (finished, miss).
+template <bool kCount>
+__device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ nodes, V3 o, V3 d,
+                                          float tmax, TraceCount* cnt) {
+  s.o = o;
+  s.d = d;
+  s.y = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  // keep the reciprocals in registers: without this barrier the compiler
+  // re-derives 1/d (a full division) at every use inside the loop
+  __asm__ volatile("" : "+v"(s.y.x), "+v"(s.y.y), "+v"(s.y.z));
+  s.fast = 0;
+  if (fabsf(d.x) >= 0x1p-60f && fabsf(d.x) <= 16.0f) s.fast |= 1u;
+  if (fabsf(d.y) >= 0x1p-60f && fabsf(d.y) <= 16.0f) s.fast |= 2u;
+  if (fabsf(d.z) >= 0x1p-60f && fabsf(d.z) <= 16.0f) s.fast |= 4u;
+  s.t = tmax;
+  s.u = s.v = 0.0f;
+  s.id = -1;
+  s.k = s.depth = s.trail = s.axis = 0;
+  if (kCount) cnt->popped += 1;  // the root
+  const float4 rs = reinterpret_cast<const float4*>(nodes)[5];  // node 0: cslab.zw, own slab
+  s.r = narrow(make_float2(kEpsilon, kInfinity), rs.z, rs.w, o.x, d.x, s.y.x, s.fast & 1u);
+  return (s.r.x <= s.r.y) && (s.r.x <= s.t);
+}
+
+// Visits node s.k; returns true when the traversal has finished.  All paths
+// (take a child, resume a deferred child, finish) leave through one exit and
+// the next node is chosen by selects: with several early exits the compiler's
+// structured control flow ran the node visit separately for lanes arriving
+// from different paths, doubling the instructions issued per visit.
+template <bool kAny, bool kCount>
+__device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ nodes,
+                                          uint32_t n_nodes, float2* stk, uint32_t stride,
+                                          TraceCount* cnt) {
+  const uint32_t k = s.k;
+  if (kCount) {
+    cnt->passed += 1;
+    cnt->popped += (2 * k + 1 < n_nodes) + (2 * k + 2 < n_nodes);
+  }
+  const float4* rec = reinterpret_cast<const float4*>(nodes + k);
+  const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
+  float u, v, t;
+  bool stop = false;  // any hit found
+  if (tri_test(s.o, s.d, mk(a0.x, a0.y, a0.z), mk(a0.w, a1.x, a1.y), mk(a1.z, a1.w, a2.x), u, v, t) &&
+      t > kEpsilon && t < s.t) {
+    s.t = t;
+    s.u = u;
+    s.v = v;
+    s.id = (int32_t)(2 * k);
+    if (kAny) stop = true;
+  }
+  if (!stop &&
+      tri_test(s.o, s.d, mk(a2.y, a2.z, a2.w), mk(a3.x, a3.y, a3.z), mk(a3.w, a4.x, a4.y), u, v, t) &&
+      t > kEpsilon && t < s.t) {
+    s.t = t;
+    s.u = u;
+    s.v = v;
+    s.id = (int32_t)(2 * k + 1);
+    if (kAny) stop = true;
+  }
+  bool found = false;
+  uint32_t nk = 0, nd = 0, na = 0;
+  float2 nrg = s.r;
+  const uint32_t cl = 2 * k + 1;
+  if (!stop && cl < n_nodes) {
+    const uint32_t cd = s.depth + 1;
+    const int ca = s.axis == 2 ? 0 : (int)s.axis + 1;
+    const float oc = compf(s.o, ca), dc = compf(s.d, ca), yc = compf(s.y, ca);
+    const bool fd = (s.fast >> ca) & 1u;
+    const float2 rl = narrow(s.r, a4.z, a4.w, oc, dc, yc, fd);
+    const bool okl = rl.x <= rl.y;
+    float2 rr = make_float2(0.0f, 0.0f);
+    bool okr = false;
+    if (cl + 1 < n_nodes) {
+      const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
+      rr = narrow(s.r, cs.x, cs.y, oc, dc, yc, fd);
+      okr = rr.x <= rr.y;
+    }
+    const bool sw = dc < 0.0f;
+    const float2 near_r = sw ? rr : rl, far_r = sw ? rl : rr;
+    const bool nok = sw ? okr : okl, fok = sw ? okl : okr;
+    const bool take_near = nok && near_r.x <= s.t;
+    const bool take_far = !take_near && fok && far_r.x <= s.t;
+    if (take_near && fok) {
+      stk[cd * stride] = far_r;
+      s.trail |= 1u << cd;
+    }
+    found = take_near || take_far;
+    nk = (take_near != sw) ? cl : cl + 1;  // near = cl unless swapped; far is the other
+    nrg = take_near ? near_r : far_r;
+    nd = cd;
+    na = (uint32_t)ca;
+  }
+  if (!stop && !found) {
+    while (s.trail) {
+      const uint32_t dd = 31u - (uint32_t)__clz(s.trail);
+      s.trail &= ~(1u << dd);
+      const float2 fr = stk[dd * stride];
+      if (fr.x <= s.t) {
+        const uint32_t anc = ((k + 1) >> (s.depth - dd)) - 1;
+        nk = (anc & 1u) ? anc + 1 : anc - 1;
+        nd = dd;
+        na = dd % 3;
+        nrg = fr;
+        found = true;
+        break;
+      }
+    }
+  }
+  s.k = nk;
+  s.depth = nd;
+  s.axis = na;
+  s.r = nrg;
+  return !found;
+}
+
+// Persistent-wave traversal of a work range [0, n).  Wave w owns
+// [w*chunk, (w+1)*chunk); between node visits, once at least kRefill lanes
+// are idle (or all are), idle lanes take the next items of the range, ranked
+// by ballot (no atomics).  fetch(i, o, d, tmax) loads item i and returns false
+// if the item needs no traversal; store(i, s) receives the finished state.
+template <bool kAny, int kRefill, class Fetch, class Store>
+__device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
+                                                 uint64_t n, float2* stk, uint32_t stride,
+                                                 Fetch fetch, Store store) {
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t chunk = (n + waves - 1) / waves;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint64_t next = wave * chunk;
+  const uint64_t end = next + chunk < n ? next + chunk : n;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  Trav s;
+  bool active = false;
+  uint64_t item = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (nidle >= (uint32_t)kRefill || nidle == 64) {
+      if (next < end) {
+        if (!active) {
+          const uint64_t i = next + __popcll(idle & lt);
+          if (i < end) {
+            V3 o, d;
+            float tmax;
+            if (fetch(i, o, d, tmax)) {
+              item = i;
+              active = trav_init<false>(s, nodes, o, d, tmax, nullptr);
+              if (!active) store(i, s);
+            }
+          }
+        }
+        next += nidle;
+      } else if (nidle == 64) {
+        break;
+      }
+    }
+    if (active && trav_step<kAny, false>(s, nodes, n_nodes, stk, stride, nullptr)) {
+      store(item, s);
+      active = false;
+    }
+  }
+}
+
+// Whole traversal of one ray by one lane (used by the render kernels).
 // stk: this lane's stack column, stk[depth * stride].
 template <bool kAny, bool kCount = false>
 __device__ __forceinline__ TraceResult trace_ray(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
                                                  V3 o, V3 d, float tmax, float2* stk,
                                                  uint32_t stride, TraceCount* cnt = nullptr) {
-  TraceResult res{-1, tmax, 0.0f, 0.0f};
-  if (kCount) cnt->popped += 1;  // the root
-  const float4 rs = reinterpret_cast<const float4*>(nodes)[5];  // node 0: cslab.zw, pad
-  float2 r = narrow(make_float2(kEpsilon, kInfinity), rs.z, rs.w, o.x, d.x);
-  if (!((r.x <= r.y) && (r.x <= res.t))) return res;
-  uint32_t k = 0, depth = 0, trail = 0;
-  for (;;) {
-    if (kCount) {
-      cnt->passed += 1;
-      cnt->popped += (2 * k + 1 < n_nodes) + (2 * k + 2 < n_nodes);
+  Trav s;
+  if (trav_init<kCount>(s, nodes, o, d, tmax, cnt)) {
+    while (!trav_step<kAny, kCount>(s, nodes, n_nodes, stk, stride, cnt)) {
     }
-    const float4* rec = reinterpret_cast<const float4*>(nodes + k);
-    const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
-    float u, v, t;
-    if (tri_test(o, d, mk(a0.x, a0.y, a0.z), mk(a0.w, a1.x, a1.y), mk(a1.z, a1.w, a2.x), u, v, t) &&
-        t > kEpsilon && t < res.t) {
-      res = TraceResult{(int32_t)(2 * k), t, u, v};
-      if (kAny) return res;
-    }
-    if (tri_test(o, d, mk(a2.y, a2.z, a2.w), mk(a3.x, a3.y, a3.z), mk(a3.w, a4.x, a4.y), u, v, t) &&
-        t > kEpsilon && t < res.t) {
-      res = TraceResult{(int32_t)(2 * k + 1), t, u, v};
-      if (kAny) return res;
-    }
-    const uint32_t cl = 2 * k + 1;
-    if (cl < n_nodes) {
-      const uint32_t cd = depth + 1;
-      const int ca = (int)(cd % 3);
-      const float oc = comp(o, ca), dc = comp(d, ca);
-      const float2 rl = narrow(r, a4.z, a4.w, oc, dc);
-      const bool okl = rl.x <= rl.y;
-      float2 rr = make_float2(0.0f, 0.0f);
-      bool okr = false;
-      if (cl + 1 < n_nodes) {
-        const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
-        rr = narrow(r, cs.x, cs.y, oc, dc);
-        okr = rr.x <= rr.y;
-      }
-      const bool sw = dc < 0.0f;
-      const float2 nr = sw ? rr : rl, fr = sw ? rl : rr;
-      const bool nok = sw ? okr : okl, fok = sw ? okl : okr;
-      const uint32_t ni = sw ? cl + 1 : cl, fi = sw ? cl : cl + 1;
-      if (nok && nr.x <= res.t) {
-        if (fok) {
-          stk[cd * stride] = fr;
-          trail |= 1u << cd;
-        }
-        k = ni;
-        r = nr;
-        depth = cd;
-        continue;
-      }
-      if (fok && fr.x <= res.t) {
-        k = fi;
-        r = fr;
-        depth = cd;
-        continue;
-      }
-    }
-    bool resumed = false;
-    while (trail) {
-      const uint32_t dd = 31u - (uint32_t)__clz(trail);
-      trail &= ~(1u << dd);
-      const float2 fr = stk[dd * stride];
-      if (fr.x <= res.t) {
-        const uint32_t anc = ((k + 1) >> (depth - dd)) - 1;
-        k = (anc & 1u) ? anc + 1 : anc - 1;
-        depth = dd;
-        r = fr;
-        resumed = true;
-        break;
-      }
-    }
-    if (!resumed) break;
   }
-  return res;
+  return TraceResult{s.id, s.t, s.u, s.v};
 }
 
 }  // namespace aur

@@ -48,6 +48,11 @@ class RenderOpts(ctypes.Structure):
     ]
 
 
+class TraceParams(ctypes.Structure):
+    _fields_ = [("persistent", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -85,6 +90,8 @@ def load():
         "aur_trace_closest": ([P, u64, P, P], i32),
         "aur_trace_any": ([P, u64, P, P], i32),
         "aur_trace_closest_stats": ([P, u64, P, P, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
+        "aur_set_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
+        "aur_get_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
         "aur_frame_create": ([P, u32, u32, u32], i32),
         "aur_frame_destroy": ([P], i32),
         "aur_frame_render": ([P, P, ctypes.POINTER(RenderOpts)], i32),
@@ -211,6 +218,16 @@ class Context:
     def trace_closest_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> None:
         self._check(self._lib.aur_trace_closest(self._h, n, ctypes.c_void_p(rays_ptr),
                                                 ctypes.c_void_p(hits_ptr)), "trace_closest")
+
+    def set_trace_params(self, persistent: int = 1, block: int = 128, refill: int = 8,
+                         waves_per_cu: int = 0) -> None:
+        p = TraceParams(persistent, block, refill, waves_per_cu)
+        self._check(self._lib.aur_set_trace_params(self._h, ctypes.byref(p)), "set_trace_params")
+
+    def get_trace_params(self) -> dict:
+        p = TraceParams()
+        self._check(self._lib.aur_get_trace_params(self._h, ctypes.byref(p)), "get_trace_params")
+        return {k: getattr(p, k) for k, _ in TraceParams._fields_}
 
     def trace_closest_stats_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> tuple[int, int]:
         p, q = ctypes.c_uint64(), ctypes.c_uint64()

@@ -1,0 +1,95 @@
+"""Autotuner for launch shapes (no reference counterpart).
+
+The reference launches fixed shapes (64-thread blocks, 65,536-ray tiles,
+src/kernels/monteCarloRaytracer.cu:119-124, src/render/monteCarloRaytracer.cpp:13).
+Here launch shapes are picked by measurement on the device at hand:
+
+* tune_trace: persistent vs flat traversal, block size, refill threshold and
+  resident waves per CU for aur_trace_closest/any, timed with HIP events on
+  the context's stream over the caller's own rays.
+
+Results are cached in a JSON file keyed by device name and workload key, so a
+later process reuses them without re-measuring.  Results never depend on the
+chosen shape (the traversal order per ray is fixed), only speed does.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "tune.json")
+
+TRACE_CANDIDATES = (
+    [dict(persistent=0, block=b, refill=1, waves_per_cu=0) for b in (64, 128, 256)]
+    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0) for b in (64, 128, 256)
+       for r in (1, 8, 16, 32)]
+    + [dict(persistent=1, block=128, refill=8, waves_per_cu=w) for w in (8, 12, 16, 24, 32)]
+)
+
+
+def _device_name() -> str:
+    import torch
+
+    return torch.cuda.get_device_name(0)
+
+
+def _load(path: str) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path: str, data: dict) -> None:
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def time_trace(ctx, n: int, rays_ptr: int, hits_ptr: int, reps: int = 3, any_hit: bool = False) -> float:
+    """Mean milliseconds of one trace launch with the context's current params."""
+    import torch
+
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    launch = ctx.trace_any_ptr if any_hit else ctx.trace_closest_ptr
+    launch(n, rays_ptr, hits_ptr)  # warm-up (code objects, caches)
+    ctx.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        launch(n, rays_ptr, hits_ptr)
+    e1.record(stream)
+    ctx.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_trace(ctx, n: int, rays_ptr: int, hits_ptr: int, key: str, reps: int = 3,
+               cache: str | None = DEFAULT_CACHE, candidates=TRACE_CANDIDATES, force: bool = False,
+               any_hit: bool = False):
+    """Sets the fastest trace params on ctx and returns (params, table).
+
+    table: list of (params, ms) for every candidate measured (empty when the
+    cached choice was reused)."""
+    full_key = f"{_device_name()}|{'any' if any_hit else 'closest'}|{key}"
+    data = _load(cache) if cache else {}
+    if not force and full_key in data:
+        p = data[full_key]["params"]
+        ctx.set_trace_params(**p)
+        return p, []
+    table = []
+    for cand in candidates:
+        ctx.set_trace_params(**cand)
+        table.append((dict(cand), time_trace(ctx, n, rays_ptr, hits_ptr, reps, any_hit)))
+    best = min(table, key=lambda t: t[1])
+    ctx.set_trace_params(**best[0])
+    if cache:
+        data[full_key] = {"params": best[0], "ms": best[1]}
+        _store(cache, data)
+    return best[0], table

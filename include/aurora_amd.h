@@ -143,6 +143,18 @@ uint32_t aur_geometry_count(const aur_context* ctx);
  * pointers; queued on the context stream. ---- */
 aur_status aur_trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits);
 aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* hit_ids);
+/* Launch shape of the trace entry points (no reference counterpart: the
+ * reference launched one thread per ray in blocks of 64,
+ * src/kernels/monteCarloRaytracer.cu:119-124).  persistent = 1: resident
+ * waves each own a contiguous ray range and refill idle lanes once `refill`
+ * (1, 8, 16 or 32) lanes are idle; waves_per_cu = 0 uses the occupancy
+ * limit.  persistent = 0: one lane per ray.  block: 64, 128 or 256.
+ * Results do not depend on these values. */
+typedef struct aur_trace_params {
+  uint32_t persistent, block, refill, waves_per_cu;
+} aur_trace_params;
+aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* params);
+aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* params);
 /* Closest hit plus the reference loop's traversal counters summed over the
  * rays: popped = states popped with index < count, passed = states whose slab
  * test passed.  Synchronous.  Used for the algorithmic-bytes model
