@@ -1,22 +1,26 @@
 """Benchmark of the render core's hot path on one MI355X.
 
-Headline (default): incoherent closest-hit traversal at 1M triangles -- the
-config of 200 subdivision-4 icospheres (1,024,002 triangles, 100 m box) with
-2^24 rays, origins uniform in the box and directions uniform on the sphere.
-A step is one launch over the 2^24 rays, already resident in HBM.  value =
-rays traced per second (Mrays/s), timed with HIP events on the library's
-stream.
+Headline (default workload "trace1m"): incoherent closest-hit traversal at
+1M triangles -- 200 subdivision-4 icospheres + ground plane (1,024,002
+triangles, 100 m box) with 2^24 rays, origins uniform in the box and
+directions uniform on the sphere.  A step is one launch over the 2^24 rays,
+already resident in HBM.  value = rays traced per second (Mrays/s), timed
+with HIP events on the library's stream.
 
 Roofline: algorithmic bytes per ray B = 32 + 16 + 24 P + 48 S (ray in, hit
 out, two triangles' axis coordinates for every popped in-range node P, their
 other twelve floats for every slab-passing node S), with P and S counted by
 an instrumented launch over the same rays.  achieved = sum(B) / mean launch
-time; peak = 8 TB/s HBM.
+time; peak = 8 TB/s HBM; traffic = calibrated rocprofv3 FETCH_SIZE +
+WRITE_SIZE bytes per launch from the committed summary.
 
 cpu_baseline: the CPU restatement of the reference traversal (the oracle)
 on a bounded sample of the same rays, all host threads.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Other workloads print their own JSON line:
+  soup10m  10M-triangle soup: NMH build ms, incoherent and coherent Mrays/s
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload trace1m]
 """
 from __future__ import annotations
 
@@ -51,7 +55,7 @@ def parse_traffic(kernel: str, summary: str):
     return None
 
 
-def cpu_baseline(tree_vertices: np.ndarray, rays: np.ndarray, budget_s: float = 15.0):
+def cpu_trace_baseline(tree_vertices: np.ndarray, rays: np.ndarray, budget_s: float = 15.0):
     from tests import oracle_ffi
 
     threads = os.cpu_count() or 1
@@ -74,41 +78,30 @@ def cpu_baseline(tree_vertices: np.ndarray, rays: np.ndarray, budget_s: float = 
     }
 
 
-def main() -> None:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rays", type=int, default=1 << 24)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
-                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
-    a = ap.parse_args()
-    if a.copy_calibration:
-        import torch
-
-        n = a.copy_calibration * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "launches": 5}))
-        return
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the render core is single-GPU; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def time_launches(ctx, launch, steps: int, warmup: int):
+    """HIP-event timing on the context's stream; returns (ms per step, list)."""
     import torch
 
-    from aurora_amd import native, scenes
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for _ in range(warmup):
+        launch()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
+        launch()
+        ev[i + 1].record(stream)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return ev[0].elapsed_time(ev[-1]) / steps, per
 
-    if not torch.cuda.is_available():
-        print("bench.py: no GPU visible", file=sys.stderr)
-        sys.exit(1)
+
+def workload_trace1m(a) -> dict:
+    import torch
+
+    from aurora_amd import native, scenes, tune
 
     t0 = time.perf_counter()
     scene = scenes.spheres()
@@ -116,7 +109,7 @@ def main() -> None:
     ctx = native.Context(0)
     ctx.upload_scene(scene)
     stats = ctx.rebuild_nmh()
-    stats = ctx.rebuild_nmh()  # second build: warm caches/allocations
+    stats = ctx.rebuild_nmh()  # second build: warm code objects and allocations
     log(f"NMH build: {stats.levels} levels, {stats.build_ms:.2f} ms")
     perm = ctx.read_permutation()
 
@@ -131,37 +124,24 @@ def main() -> None:
     bytes_per_launch = 48 * a.rays + 24 * popped + 48 * passed
     hit_rate = float((hits[:, 0].view(torch.int32) >= 0).float().mean().item())
     log(f"P={popped / a.rays:.2f} S={passed / a.rays:.2f} per ray; hit rate {hit_rate:.3f}")
+    if a.tune:
+        params, _ = tune.tune_trace(ctx, a.rays, rays.data_ptr(), hits.data_ptr(), key="trace1m")
+        log(f"tuned trace params: {params}")
+    params = ctx.get_trace_params()
 
-    stream = torch.cuda.ExternalStream(ctx.stream)
-    for _ in range(a.warmup):
-        ctx.trace_closest_ptr(a.rays, rays.data_ptr(), hits.data_ptr())
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
-    wall0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(a.steps):
-        ctx.trace_closest_ptr(a.rays, rays.data_ptr(), hits.data_ptr())
-        ev[i + 1].record(stream)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(a.steps)]
-    ms_total = ev[0].elapsed_time(ev[-1])
-    ms_step = ms_total / a.steps
-    mrays = a.rays * a.steps / (ms_total * 1e-3) / 1e6
+    launch = lambda: ctx.trace_closest_ptr(a.rays, rays.data_ptr(), hits.data_ptr())  # noqa: E731
+    ms_step, per = time_launches(ctx, launch, a.steps, a.warmup)
+    mrays = a.rays / (ms_step * 1e-3) / 1e6
     achieved = bytes_per_launch / (ms_step * 1e-3) / 1e9
-    log(f"{ms_step:.3f} ms/launch (median {np.median(per):.3f}), {mrays:.1f} Mrays/s, "
-        f"wall {wall:.2f}s")
+    log(f"{ms_step:.3f} ms/launch (median {np.median(per):.3f}), {mrays:.1f} Mrays/s")
 
     cpu = None
     if not a.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(scene.vertices[perm], rays_np)
+        cpu = cpu_trace_baseline(scene.vertices[perm], rays_np)
         log(f"cpu baseline {cpu['value']} Mrays/s on {cpu['cores']} threads")
-
-    traffic = parse_traffic("k_trace", "k_trace_incoherent_1M.md")
-    out = {
+    ctx.close()
+    return {
         "metric": "Mrays/s @1M tri (incoherent closest hit)",
         "value": round(mrays, 2),
         "unit": "Mrays/s",
@@ -183,6 +163,7 @@ def main() -> None:
             "popped_per_ray": round(popped / a.rays, 3),
             "passed_per_ray": round(passed / a.rays, 3),
             "hit_rate": round(hit_rate, 4),
+            "trace_params": params,
         },
         "roofline": {
             "bound": "hbm",
@@ -190,12 +171,94 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": parse_traffic("k_trace_pt", "k_trace_incoherent_1M.md"),
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+
+
+def workload_soup10m(a) -> dict:
+    import torch
+
+    from aurora_amd import native, scenes
+
+    t0 = time.perf_counter()
+    scene = scenes.soup(n=10_000_000)
+    log(f"soup: {scene.count} triangles ({time.perf_counter() - t0:.1f}s)")
+    ctx = native.Context(0)
+    ctx.upload_scene(scene)
+    builds = [ctx.rebuild_nmh().build_ms for _ in range(3)]
+    log(f"NMH build ms: {builds}")
+    out = {}
+    for name, rays_np in (("incoherent", scenes.incoherent_rays(1 << 24, 0.0, 1000.0, seed=33)),
+                          ("coherent", scenes.coherent_rays(scene.camera, 4096, 4096))):
+        rays = torch.from_numpy(rays_np).to("cuda")
+        hits = torch.empty((rays.shape[0], 4), dtype=torch.float32, device="cuda")
+        n = rays.shape[0]
+        ms, _ = time_launches(ctx, lambda: ctx.trace_closest_ptr(n, rays.data_ptr(), hits.data_ptr()),
+                              a.steps, a.warmup)
+        out[name] = round(n / (ms * 1e-3) / 1e6, 1)
+        log(f"{name}: {ms:.3f} ms, {out[name]} Mrays/s")
+    cpu = None
+    if not a.no_cpu:
+        from tests import oracle_ffi
+
+        sub = scene.vertices[: 1 << 21]
+        t0 = time.perf_counter()
+        oracle_ffi.nmh_build(sub, threads=1)
+        cpu = {"value": round((time.perf_counter() - t0) * 1e3, 1), "unit": "ms", "cores": 1,
+               "kind": "port",
+               "sample": "literal NMH build of the first 2,097,152 soup triangles, one core"}
     ctx.close()
+    return {
+        "metric": "NMH build ms @10M tri", "value": round(float(np.median(builds[1:])), 3),
+        "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded 10M-triangle soup)",
+        "config": {"workload": "10M-triangle soup build + 2^24 incoherent + 4096^2 coherent rays",
+                   "triangles": scene.count, "incoherent_mrays": out["incoherent"],
+                   "coherent_mrays": out["coherent"]},
+        "cpu_baseline": cpu,
+    }
+
+
+WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--workload", default="trace1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--tune", action="store_true", help="autotune trace launch params first")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "launches": 5}))
+        return
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the render core runs on a single GPU; DESIGN.md describes how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    print(json.dumps(WORKLOADS[a.workload](a)), flush=True)
 
 
 if __name__ == "__main__":

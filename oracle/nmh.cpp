@@ -14,10 +14,13 @@
 // thread-per-split kernel (first maximum by position wins) is used at every
 // level.  Keys are compared as floats with '<' (vertex data is canonicalised
 // so that -0.0 does not occur; NaN is rejected by the caller).
+#include <omp.h>
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <parallel/algorithm>
 #include <vector>
 
 #include "oracle_math.h"
@@ -40,16 +43,26 @@ void calc_partition(uint32_t N, uint32_t& L, uint32_t& R) {
   L = 2 * (n - 1) - R;
 }
 
+// Stable sort, serial or on OpenMP threads (libstdc++ parallel mode); both
+// produce the unique stable order.
+template <class Cmp>
+void stable_sort_n(std::vector<uint32_t>& v, Cmp cmp, int threads) {
+  if (threads > 1 && v.size() > 65536) {
+    omp_set_num_threads(threads);
+    __gnu_parallel::stable_sort(v.begin(), v.end(), cmp);
+  } else {
+    std::stable_sort(v.begin(), v.end(), cmp);
+  }
+}
+
 // sortTriangles: permutation = stable sort by key, then stable sort by node.
 void sort_triangles(uint32_t count, const float* keys, std::vector<uint32_t>& indices,
                     std::vector<uint32_t>& nodes, std::vector<uint32_t>& perm,
-                    std::vector<uint32_t>& tmp) {
+                    std::vector<uint32_t>& tmp, int threads) {
   std::iota(perm.begin(), perm.end(), 0u);
-  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
-    return keys[indices[a]] < keys[indices[b]];
-  });
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](uint32_t a, uint32_t b) { return nodes[a] < nodes[b]; });
+  stable_sort_n(perm, [&](uint32_t a, uint32_t b) { return keys[indices[a]] < keys[indices[b]]; },
+                threads);
+  stable_sort_n(perm, [&](uint32_t a, uint32_t b) { return nodes[a] < nodes[b]; }, threads);
   for (uint32_t i = 0; i < count; ++i) tmp[i] = indices[perm[i]];
   indices.swap(tmp);
   for (uint32_t i = 0; i < count; ++i) tmp[i] = nodes[perm[i]];
@@ -68,10 +81,12 @@ uint32_t orc_nmh_levels(uint32_t count) { return orc::log2i(count / 2) + 1; }
 
 // cudaRebuildNMH restated.  vertices: count triangles in "points" layout
 // (x1 x2 x3 y1 y2 y3 z1 z2 z3).  indices_out[i] = original triangle index at
-// tree position i.  Returns 0 on success, -1 on invalid input, -2 if an
-// internal consistency check fails (a split slot read that this level did
-// not write, which the reference would read as stale memory).
-int orc_nmh_build(uint32_t count, const float* vertices, uint32_t* indices_out) {
+// tree position i.  threads > 1 runs the two stable sorts of every level on
+// that many OpenMP threads (same result).  Returns 0 on success, -1 on
+// invalid input, -2 if an internal consistency check fails (a split slot
+// read that this level did not write, which the reference would read as
+// stale memory).
+int orc_nmh_build_mt(uint32_t count, const float* vertices, uint32_t* indices_out, int threads) {
   if (count < 2 || (count & 1u)) return -1;
   const uint32_t numLevels = orc::log2i(count / 2) + 1;
   const size_t maxSplits = size_t(1) << numLevels;
@@ -102,7 +117,7 @@ int orc_nmh_build(uint32_t count, const float* vertices, uint32_t* indices_out) 
   for (uint32_t i = 0; i < numLevels; ++i) {
     const float* keysMin = vmin[axis].data();
     const float* keysMax = vmax[axis].data();
-    sort_triangles(count, keysMin, indices, nodes, perm, tmp);
+    sort_triangles(count, keysMin, indices, nodes, perm, tmp, threads);
     if (i == numLevels - 1) break;
 
     // swapMaxTriangleTPS (first maximum by position from slot +1)
@@ -162,6 +177,10 @@ int orc_nmh_build(uint32_t count, const float* vertices, uint32_t* indices_out) 
   }
   std::memcpy(indices_out, indices.data(), sizeof(uint32_t) * count);
   return 0;
+}
+
+int orc_nmh_build(uint32_t count, const float* vertices, uint32_t* indices_out) {
+  return orc_nmh_build_mt(count, vertices, indices_out, 1);
 }
 
 }  // extern "C"

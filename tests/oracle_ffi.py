@@ -32,6 +32,8 @@ def lib():
         u32, i32, u64 = ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
         L.orc_nmh_build.argtypes = [u32, P, P]
         L.orc_nmh_build.restype = i32
+        L.orc_nmh_build_mt.argtypes = [u32, P, P, i32]
+        L.orc_nmh_build_mt.restype = i32
         L.orc_nmh_levels.argtypes = [u32]
         L.orc_nmh_levels.restype = u32
         L.orc_calc_partition.argtypes = [u32, P, P]
@@ -64,11 +66,13 @@ def nmh_levels(count: int) -> int:
     return int(lib().orc_nmh_levels(count))
 
 
-def nmh_build(vertices: np.ndarray) -> np.ndarray:
+def nmh_build(vertices: np.ndarray, threads: int = 1) -> np.ndarray:
+    """Literal restatement of the reference build; threads > 1 parallelises
+    its stable sorts (identical result)."""
     v = np.ascontiguousarray(vertices, dtype=np.float32)
     n = v.shape[0]
     out = np.empty(n, dtype=np.uint32)
-    rc = lib().orc_nmh_build(n, _p(v), _p(out))
+    rc = lib().orc_nmh_build_mt(n, _p(v), _p(out), threads)
     if rc != 0:
         raise RuntimeError(f"orc_nmh_build failed: {rc}")
     return out
