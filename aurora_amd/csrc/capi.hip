@@ -59,6 +59,7 @@ aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* p) {
   if (p->persistent && p->refill != 1 && p->refill != 8 && p->refill != 16 && p->refill != 32)
     return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: refill must be 1, 8, 16 or 32");
   if (p->waves_per_cu > 64) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: waves_per_cu > 64");
+  if (p->stack_lds_rows > 32) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: stack_lds_rows > 32");
   ctx->trace_params = *p;
   return AUR_OK;
 }
@@ -66,6 +67,20 @@ aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* p) {
 aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* p) {
   if (!ctx || !p) return AUR_ERR_INVALID_ARG;
   *p = ctx->trace_params;
+  return AUR_OK;
+}
+
+aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* p) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (!p) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: null");
+  if (p->max_paths > (1u << 26)) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: max_paths > 2^26");
+  ctx->render_params.max_paths = p->max_paths ? p->max_paths : aur::kDefaultMaxPaths;
+  return AUR_OK;
+}
+
+aur_status aur_get_render_params(aur_context* ctx, aur_render_params* p) {
+  if (!ctx || !p) return AUR_ERR_INVALID_ARG;
+  *p = ctx->render_params;
   return AUR_OK;
 }
 
@@ -156,6 +171,10 @@ aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* 
     AUR_HIP(ctx, hipMemcpy(ctx->lights.p, lights, count * sizeof(aur_light), hipMemcpyHostToDevice));
   }
   ctx->num_lights = count;
+  // next-event slots per path vertex: area lights use `samples` slots, others one
+  ctx->num_slots = 0;
+  for (uint32_t i = 0; i < count; ++i)
+    ctx->num_slots += lights[i].type == AUR_LIGHT_AREA ? (uint32_t)std::max<int>(1, lights[i].samples) : 1u;
   return AUR_OK;
 }
 

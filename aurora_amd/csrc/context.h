@@ -43,10 +43,14 @@ struct DevBuf {
   }
 };
 
+// Default paths in flight per PBR wavefront pass (2^21 paths: about 0.2 GB of
+// path state plus 48 B per next-event slot and path).
+constexpr uint32_t kDefaultMaxPaths = 1u << 21;
+
 struct FrameState {
   uint32_t width = 0, height = 0, spp = 0;
   DevBuf<float4> pixels;  // RGBA accumulation (framebuffer)
-  // PBR wavefront state (one path per pixel per sample pass)
+  // PBR wavefront state, indexed by path (pixel + pixels * sample-in-pass)
   DevBuf<float4> ray_o, ray_d, beta, radiance;
   DevBuf<aur_hit> hits;
   DevBuf<uint32_t> queue[2];
@@ -66,7 +70,9 @@ struct aur_context {
   uint32_t num_cus = 256;
   // persistent waves, 256-thread blocks, refill at 16 idle lanes, occupancy
   // limited residency: fastest shape measured at 1M triangles / 2^24 rays
-  aur_trace_params trace_params{1, 256, 16, 0};
+  aur_trace_params trace_params{1, 256, 16, 0, 0};
+  aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
+  aur_render_params render_params{aur::kDefaultMaxPaths};
 
   // uploaded geometry (input order)
   uint32_t count = 0;
@@ -88,6 +94,7 @@ struct aur_context {
   uint32_t num_shaders = 0;
   aur::DevBuf<aur_light> lights;
   uint32_t num_lights = 0;
+  uint32_t num_slots = 0;  // next-event slots per path vertex
 
   aur::FrameState frame;
 

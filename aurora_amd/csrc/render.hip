@@ -300,8 +300,11 @@ __device__ inline void onb(V3 n, V3& t1, V3& t2) {
   t2 = mk(b, sign + n.y * n.y * a, -n.y);
 }
 
+// One pass traces `batch` consecutive samples of every pixel: path q is pixel
+// q % npix, sample sample + q / npix.  Per-path arrays are indexed by q.
 struct PbrArgs {
   uint32_t w, h, spp, max_depth, rr_start, seed, sample, nslots;
+  uint32_t npix, batch;
   float4 *ray_o, *ray_d, *beta, *radiance;
   aur_hit* hits;
   float4 *sh_o, *sh_d, *sh_c;
@@ -310,22 +313,29 @@ struct PbrArgs {
   unsigned long long* stats;
 };
 
+__device__ __forceinline__ uint32_t path_pixel(const PbrArgs& a, uint32_t q) { return q % a.npix; }
+__device__ __forceinline__ uint32_t path_sample(const PbrArgs& a, uint32_t q) {
+  return a.sample + q / a.npix;
+}
+
 __global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ queue) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {
-    a.counters[0] = a.w * a.h;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t npaths = a.npix * a.batch;
+  if (q == 0) {
+    a.counters[0] = npaths;
     a.counters[1] = 0;
   }
-  if (p >= a.w * a.h) return;
+  if (q >= npaths) return;
+  const uint32_t p = path_pixel(a, q), smp = path_sample(a, q);
   const uint32_t x = p % a.w, y = p / a.w;
-  const U4 r = pbr_draw(a.seed, p, a.sample, 0, 0);
+  const U4 r = pbr_draw(a.seed, p, smp, 0, 0);
   const float j0 = u01(r.x), j1 = u01(r.y);
   uint32_t sw = 1;
   while ((sw + 1) * (sw + 1) <= a.spp) ++sw;
   float fx, fy;
   if (sw * sw == a.spp) {
-    fx = ((float)(a.sample % sw) + j0) / (float)sw;
-    fy = ((float)(a.sample / sw) + j1) / (float)sw;
+    fx = ((float)(smp % sw) + j0) / (float)sw;
+    fy = ((float)(smp / sw) + j1) / (float)sw;
   } else {
     fx = j0;
     fy = j1;
@@ -334,11 +344,11 @@ __global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ q
   const float py = ((float)y + fy) / (float)a.h - 0.5f;
   const V3 d = normalize(px * ld3(cam.right) * cam.tanfov[0] + py * ld3(cam.up) * cam.tanfov[1] +
                          ld3(cam.forward));
-  a.ray_o[p] = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
-  a.ray_d[p] = make_float4(d.x, d.y, d.z, 0.0f);
-  a.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  a.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  queue[p] = p;
+  a.ray_o[q] = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
+  a.ray_d[q] = make_float4(d.x, d.y, d.z, 0.0f);
+  a.beta[q] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  a.radiance[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  queue[q] = q;
 }
 
 // Closest hit for every queued path: persistent waves over the queue.
@@ -353,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, i
     atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
   persistent_trace<false, kRefill>(
-      sc.nodes, sc.n_nodes, n_in, stk_mem + threadIdx.x, blockDim.x,
+      sc.nodes, sc.n_nodes, n_in, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t j, V3& o, V3& d, float& tmax) {
         const uint32_t p = queue[j];
         const float4 o4 = a.ray_o[p], d4 = a.ray_d[p];
@@ -370,7 +380,8 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_in = a.counters[cin];
   if (j >= n_in) return;
-  const uint32_t p = qin[j];
+  const uint32_t p = qin[j];  // path index
+  const uint32_t pix = path_pixel(a, p), smp = path_sample(a, p);
   const size_t base = (size_t)j * a.nslots;
   const aur_hit hit = a.hits[p];
   if (hit.id < 0) {
@@ -399,7 +410,7 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
       const int ns = Lt.samples > 0 ? Lt.samples : 1;
       const float inv = 1.0f / (float)ns;
       for (int k = 0; k < ns; ++k, ++e) {
-        const U4 r = pbr_draw(a.seed, p, a.sample, bounce, slot++);
+        const U4 r = pbr_draw(a.seed, pix, smp, bounce, slot++);
         const V3 Q = ld3(Lt.position) + ((ld3(Lt.e1) * u01(r.x)) + (ld3(Lt.e2) * u01(r.y)));
         V3 wi = Q - P;
         const float dist = length(wi);
@@ -451,7 +462,7 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
     }
   }
   if (bounce + 1 >= a.max_depth) return;
-  const U4 r = pbr_draw(a.seed, p, a.sample, bounce, 1);
+  const U4 r = pbr_draw(a.seed, pix, smp, bounce, 1);
   if (bounce >= a.rr_start) {
     const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
     if (u01(r.z) >= q) return;
@@ -484,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, SceneDev sc, i
   const uint64_t total = (uint64_t)a.counters[cin] * a.nslots;
   uint32_t traced = 0;
   persistent_trace<true, kRefill>(
-      sc.nodes, sc.n_nodes, total, stk_mem + threadIdx.x, blockDim.x,
+      sc.nodes, sc.n_nodes, total, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t e, V3& o, V3& d, float& tmax) {
         const float4 d4 = a.sh_d[e];
         if (d4.w != 1.0f) return false;
@@ -517,19 +528,23 @@ __global__ void k_pbr_resolve(PbrArgs a, int cin, const uint32_t* __restrict__ q
   a.radiance[p] = make_float4(L.x, L.y, L.z, L4.w);
 }
 
+// Adds the pass's samples to each pixel in sample order, so the sum does not
+// depend on how samples were grouped into passes.
 __global__ void k_pbr_accumulate(PbrArgs a, float4* __restrict__ pixels) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.w * a.h) return;
-  const float4 L = a.radiance[p];
-  if (L.w != 0.0f) {
-    const float weight = 1.0f / (float)a.spp;
-    float4 px = pixels[p];
-    px.x += L.x * weight;
-    px.y += L.y * weight;
-    px.z += L.z * weight;
-    px.w += weight;
-    pixels[p] = px;
+  if (p >= a.npix) return;
+  const float weight = 1.0f / (float)a.spp;
+  float4 px = pixels[p];
+  for (uint32_t b = 0; b < a.batch; ++b) {
+    const float4 L = a.radiance[p + (size_t)b * a.npix];
+    if (L.w != 0.0f) {
+      px.x += L.x * weight;
+      px.y += L.y * weight;
+      px.z += L.z * weight;
+      px.w += weight;
+    }
   }
+  pixels[p] = px;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
@@ -577,44 +592,45 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
     }
     AUR_LAUNCH_CHECK(ctx, "k_compat");
   } else {
-    uint32_t nslots = 0;
-    {
-      // slot count per path: area lights use `samples` slots, others one
-      std::vector<aur_light> hl(ctx->num_lights);
-      if (ctx->num_lights)
-        AUR_HIP(ctx, hipMemcpy(hl.data(), ctx->lights.p, sizeof(aur_light) * ctx->num_lights,
-                               hipMemcpyDeviceToHost));
-      for (const aur_light& l : hl) nslots += l.type == AUR_LIGHT_AREA ? (uint32_t)std::max<int>(1, l.samples) : 1u;
-    }
-    const size_t nsh = std::max<size_t>(1, (size_t)npix * nslots);
-    AUR_HIP(ctx, fr.ray_o.reserve(npix));
-    AUR_HIP(ctx, fr.ray_d.reserve(npix));
-    AUR_HIP(ctx, fr.beta.reserve(npix));
-    AUR_HIP(ctx, fr.radiance.reserve(npix));
-    AUR_HIP(ctx, fr.hits.reserve(npix));
-    AUR_HIP(ctx, fr.queue[0].reserve(npix));
-    AUR_HIP(ctx, fr.queue[1].reserve(npix));
+    const uint32_t nslots = ctx->num_slots;
+    // samples per pass: enough paths in flight to fill the device
+    const uint32_t max_paths = std::max<uint32_t>(ctx->render_params.max_paths, 1u);
+    const uint32_t batch_cap = std::max<uint32_t>(1u, max_paths / npix);
+    const uint32_t batch = std::min<uint32_t>(batch_cap, std::max<uint32_t>(1u, s1 - s0));
+    const uint64_t npaths = (uint64_t)npix * batch;
+    if (npaths * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull)
+      return ctx->fail(AUR_ERR_INVALID_ARG, "render: frame too large");
+    const size_t nsh = std::max<size_t>(1, (size_t)npaths * nslots);
+    AUR_HIP(ctx, fr.ray_o.reserve(npaths));
+    AUR_HIP(ctx, fr.ray_d.reserve(npaths));
+    AUR_HIP(ctx, fr.beta.reserve(npaths));
+    AUR_HIP(ctx, fr.radiance.reserve(npaths));
+    AUR_HIP(ctx, fr.hits.reserve(npaths));
+    AUR_HIP(ctx, fr.queue[0].reserve(npaths));
+    AUR_HIP(ctx, fr.queue[1].reserve(npaths));
     AUR_HIP(ctx, fr.sh_o.reserve(nsh));
     AUR_HIP(ctx, fr.sh_d.reserve(nsh));
     AUR_HIP(ctx, fr.sh_c.reserve(nsh));
     AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
-    PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots,
+    PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots, npix, batch,
               fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
               fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats};
-    const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npix);
-    const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, (uint64_t)npix * nslots);
-    for (uint32_t smp = s0; smp < s1; ++smp) {
+    const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npaths);
+    const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, npaths * nslots);
+    for (uint32_t smp = s0; smp < s1; smp += a.batch) {
       a.sample = smp;
-      k_pbr_camera<<<nblocks(npix, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
+      a.batch = std::min<uint32_t>(batch, s1 - smp);
+      const uint64_t np = (uint64_t)npix * a.batch;
+      k_pbr_camera<<<nblocks(np, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
       int cin = 0;
       for (uint32_t b = 0; b < o->max_depth; ++b) {
         uint32_t* qin = fr.queue[cin].p;
         uint32_t* qout = fr.queue[cin ^ 1].p;
         k_pbr_extend<<<grid_extend, kBlock, lds, s>>>(a, sc, cin, qin);
-        k_pbr_shade<<<nblocks(npix, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
+        k_pbr_shade<<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
         if (nslots) {
           k_pbr_shadow<<<grid_shadow, kBlock, lds, s>>>(a, sc, cin);
-          k_pbr_resolve<<<nblocks(npix, 256), 256, 0, s>>>(a, cin, qin);
+          k_pbr_resolve<<<nblocks(np, 256), 256, 0, s>>>(a, cin, qin);
         }
         cin ^= 1;
       }

@@ -33,10 +33,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                                                         uint32_t n_nodes,
                                                         const aur_ray* __restrict__ rays, uint64_t n,
                                                         aur_hit* __restrict__ hits,
-                                                        int32_t* __restrict__ ids) {
+                                                        int32_t* __restrict__ ids,
+                                                        float2* __restrict__ gstack, uint32_t dlo) {
   extern __shared__ float2 stk_mem[];
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const TravStack stk = dlo ? TravStack{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
+                                        (uint64_t)gridDim.x * blockDim.x, dlo, true}
+                            : lds_stack(stk_mem + threadIdx.x, blockDim.x);
   persistent_trace<kAny, kRefill>(
-      nodes, n_nodes, n, stk_mem + threadIdx.x, blockDim.x,
+      nodes, n_nodes, n, stk,
       [&](uint64_t i, V3& o, V3& d, float& tmax) {
         const float4* r = reinterpret_cast<const float4*>(rays + i);
         const float4 a = r[0], b = r[1];
@@ -61,7 +66,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restr
   const float4 a = r[0], b = r[1];
   Trav s;
   if (trav_init<false>(s, nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
-    while (!trav_step<kAny, false>(s, nodes, n_nodes, stk_mem + threadIdx.x, blockDim.x, nullptr)) {
+    const TravStack stk = lds_stack(stk_mem + threadIdx.x, blockDim.x);
+    while (!trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
     }
   }
   store_result(i, s, hits, ids);
@@ -112,6 +118,11 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     AUR_LAUNCH_CHECK(ctx, "k_trace_flat");
     return AUR_OK;
   }
+  // stack rows: the deepest `rows` depths in LDS, the rest in global memory
+  const uint32_t L = ctx->levels;
+  const uint32_t rows = (tp.stack_lds_rows == 0 || tp.stack_lds_rows >= L) ? L : tp.stack_lds_rows;
+  const uint32_t dlo = L - rows;
+  const size_t lds_pt = (size_t)rows * block * sizeof(float2);
   // resident waves: occupancy-limited blocks per CU times CUs (or the request)
   int per_cu = 0;
   const void* fn = nullptr;
@@ -122,7 +133,7 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     case 32: fn = (const void*)k_trace_pt<kAny, 32>; break;
     default: return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be 1, 8, 16 or 32");
   }
-  AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds));
+  AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds_pt));
   if (per_cu < 1) per_cu = 1;
   uint64_t blocks_total = (uint64_t)per_cu * ctx->num_cus;
   if (tp.waves_per_cu) blocks_total = std::max<uint64_t>(1, (uint64_t)tp.waves_per_cu * ctx->num_cus * 64 / block);
@@ -130,11 +141,16 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   uint64_t nwaves = blocks_total * block / 64;
   if (nwaves > max_waves) nwaves = max_waves;
   const uint32_t grid = (uint32_t)((nwaves * 64 + block - 1) / block);
+  float2* gstack = nullptr;
+  if (dlo) {
+    AUR_HIP(ctx, ctx->gstack.reserve((size_t)dlo * grid * block));
+    gstack = ctx->gstack.p;
+  }
   switch (tp.refill) {
-    case 1: k_trace_pt<kAny, 1><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
-    case 8: k_trace_pt<kAny, 8><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
-    case 16: k_trace_pt<kAny, 16><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
-    default: k_trace_pt<kAny, 32><<<grid, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids); break;
+    case 1: k_trace_pt<kAny, 1><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
+    case 8: k_trace_pt<kAny, 8><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
+    case 16: k_trace_pt<kAny, 16><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
+    default: k_trace_pt<kAny, 32><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
   }
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;

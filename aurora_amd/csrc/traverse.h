@@ -92,6 +92,36 @@ struct Trav {
 
 __device__ __forceinline__ float compf(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
+// Per-lane stack of deferred ranges, one row per tree depth.  Rows for depth
+// >= dlo live in LDS (lane-strided, conflict-free); shallower rows live in a
+// per-lane column of global memory.  Deep entries are pushed and popped at
+// almost every visit, shallow ones a few times per ray, so keeping only the
+// deep rows in LDS frees LDS for more resident waves.
+// Measured at 1M triangles the split is slower than keeping every row in LDS
+// (occupancy is not what limits the kernel), so it is a tuning option only.
+struct TravStack {
+  float2* lds;        // row r of this lane at lds[r * lstride]
+  uint32_t lstride;
+  float2* glb;        // row d of this lane at glb[d * gstride]; null: all rows in LDS
+  uint64_t gstride;
+  uint32_t dlo;
+  bool split;
+  __device__ __forceinline__ void store(uint32_t d, float2 v) const {
+    if (!split || d >= dlo)
+      lds[(d - dlo) * lstride] = v;
+    else
+      glb[d * gstride] = v;
+  }
+  __device__ __forceinline__ float2 load(uint32_t d) const {
+    return (!split || d >= dlo) ? lds[(d - dlo) * lstride] : glb[d * gstride];
+  }
+};
+
+// All rows in LDS.
+__device__ __forceinline__ TravStack lds_stack(float2* lane_column, uint32_t stride) {
+  return TravStack{lane_column, stride, nullptr, 0, 0, false};
+}
+
 // Sets up a ray; returns false when the root's slab culls it (finished, miss).
 template <bool kCount>
 __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ nodes, V3 o, V3 d,
@@ -123,7 +153,7 @@ __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ n
 // from different paths, doubling the instructions issued per visit.
 template <bool kAny, bool kCount>
 __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ nodes,
-                                          uint32_t n_nodes, float2* stk, uint32_t stride,
+                                          uint32_t n_nodes, const TravStack& stk,
                                           TraceCount* cnt) {
   const uint32_t k = s.k;
   if (kCount) {
@@ -175,7 +205,7 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     const bool take_near = nok && near_r.x <= s.t;
     const bool take_far = !take_near && fok && far_r.x <= s.t;
     if (take_near && fok) {
-      stk[cd * stride] = far_r;
+      stk.store(cd, far_r);
       s.trail |= 1u << cd;
     }
     found = take_near || take_far;
@@ -188,7 +218,7 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     while (s.trail) {
       const uint32_t dd = 31u - (uint32_t)__clz(s.trail);
       s.trail &= ~(1u << dd);
-      const float2 fr = stk[dd * stride];
+      const float2 fr = stk.load(dd);
       if (fr.x <= s.t) {
         const uint32_t anc = ((k + 1) >> (s.depth - dd)) - 1;
         nk = (anc & 1u) ? anc + 1 : anc - 1;
@@ -214,8 +244,8 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
 // if the item needs no traversal; store(i, s) receives the finished state.
 template <bool kAny, int kRefill, class Fetch, class Store>
 __device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
-                                                 uint64_t n, float2* stk, uint32_t stride,
-                                                 Fetch fetch, Store store) {
+                                                 uint64_t n, const TravStack& stk, Fetch fetch,
+                                                 Store store) {
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t chunk = (n + waves - 1) / waves;
   const uint32_t lane = threadIdx.x & 63u;
@@ -248,7 +278,7 @@ __device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nod
         break;
       }
     }
-    if (active && trav_step<kAny, false>(s, nodes, n_nodes, stk, stride, nullptr)) {
+    if (active && trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
       store(item, s);
       active = false;
     }
@@ -262,8 +292,9 @@ __device__ __forceinline__ TraceResult trace_ray(const NodeRec* __restrict__ nod
                                                  V3 o, V3 d, float tmax, float2* stk,
                                                  uint32_t stride, TraceCount* cnt = nullptr) {
   Trav s;
+  const TravStack st = lds_stack(stk, stride);
   if (trav_init<kCount>(s, nodes, o, d, tmax, cnt)) {
-    while (!trav_step<kAny, kCount>(s, nodes, n_nodes, stk, stride, cnt)) {
+    while (!trav_step<kAny, kCount>(s, nodes, n_nodes, st, cnt)) {
     }
   }
   return TraceResult{s.id, s.t, s.u, s.v};

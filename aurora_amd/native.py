@@ -50,7 +50,12 @@ class RenderOpts(ctypes.Structure):
 
 class TraceParams(ctypes.Structure):
     _fields_ = [("persistent", ctypes.c_uint32), ("block", ctypes.c_uint32),
-                ("refill", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32)]
+                ("refill", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("stack_lds_rows", ctypes.c_uint32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("max_paths", ctypes.c_uint32)]
 
 
 _lib = None
@@ -92,6 +97,8 @@ def load():
         "aur_trace_closest_stats": ([P, u64, P, P, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
         "aur_set_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
         "aur_get_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
+        "aur_set_render_params": ([P, ctypes.POINTER(RenderParams)], i32),
+        "aur_get_render_params": ([P, ctypes.POINTER(RenderParams)], i32),
         "aur_frame_create": ([P, u32, u32, u32], i32),
         "aur_frame_destroy": ([P], i32),
         "aur_frame_render": ([P, P, ctypes.POINTER(RenderOpts)], i32),
@@ -219,15 +226,24 @@ class Context:
         self._check(self._lib.aur_trace_closest(self._h, n, ctypes.c_void_p(rays_ptr),
                                                 ctypes.c_void_p(hits_ptr)), "trace_closest")
 
-    def set_trace_params(self, persistent: int = 1, block: int = 128, refill: int = 8,
-                         waves_per_cu: int = 0) -> None:
-        p = TraceParams(persistent, block, refill, waves_per_cu)
+    def set_trace_params(self, persistent: int = 1, block: int = 256, refill: int = 16,
+                         waves_per_cu: int = 0, stack_lds_rows: int = 0) -> None:
+        p = TraceParams(persistent, block, refill, waves_per_cu, stack_lds_rows)
         self._check(self._lib.aur_set_trace_params(self._h, ctypes.byref(p)), "set_trace_params")
 
     def get_trace_params(self) -> dict:
         p = TraceParams()
         self._check(self._lib.aur_get_trace_params(self._h, ctypes.byref(p)), "get_trace_params")
         return {k: getattr(p, k) for k, _ in TraceParams._fields_}
+
+    def set_render_params(self, max_paths: int = 0) -> None:
+        p = RenderParams(max_paths)
+        self._check(self._lib.aur_set_render_params(self._h, ctypes.byref(p)), "set_render_params")
+
+    def get_render_params(self) -> dict:
+        p = RenderParams()
+        self._check(self._lib.aur_get_render_params(self._h, ctypes.byref(p)), "get_render_params")
+        return {"max_paths": p.max_paths}
 
     def trace_closest_stats_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> tuple[int, int]:
         p, q = ctypes.c_uint64(), ctypes.c_uint64()

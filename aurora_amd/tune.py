@@ -20,10 +20,9 @@ import os
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "tune.json")
 
 TRACE_CANDIDATES = (
-    [dict(persistent=0, block=b, refill=1, waves_per_cu=0) for b in (64, 128, 256)]
-    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0) for b in (64, 128, 256)
-       for r in (1, 8, 16, 32)]
-    + [dict(persistent=1, block=128, refill=8, waves_per_cu=w) for w in (8, 12, 16, 24, 32)]
+    [dict(persistent=0, block=b, refill=1, waves_per_cu=0, stack_lds_rows=0) for b in (64, 256)]
+    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0, stack_lds_rows=s) for b in (128, 256)
+       for r in (8, 16, 32) for s in (0, 12)]
 )
 
 

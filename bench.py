@@ -221,7 +221,102 @@ def workload_soup10m(a) -> dict:
     }
 
 
-WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m}
+def two_seed_rel_rmse(img_a: np.ndarray, img_b: np.ndarray) -> float:
+    """rel-RMSE of one n-spp render against the converged image, estimated
+    from two independent n-spp renders: E||I_a - I_b||^2 = 2 E||I - mu||^2, so
+    ||I_a - I_b|| / (sqrt(2) ||(I_a + I_b) / 2||).  Needs no reference render
+    (a finite-spp reference would add its own variance to the distance)."""
+    a = img_a[:, :3].astype(np.float64)
+    b = img_b[:, :3].astype(np.float64)
+    d = a - b
+    m = 0.5 * (a + b)
+    return float(np.sqrt((d * d).sum()) / (np.sqrt(2.0) * np.sqrt((m * m).sum())))
+
+
+def render_convergence(a, scene, label: str, max_spp: int) -> dict:
+    """Wall seconds to 1% rel-RMSE for the PBR path loop: renders of 1, 2, 4,
+    ... spp (seed 7, timed from the call to the image in host memory), each
+    with a second render at seed 8 for the two-seed error estimate, until the
+    estimate is <= 1%."""
+    from aurora_amd import native
+
+    ctx = native.Context(0)
+    ctx.upload_scene(scene)
+    ctx.rebuild_nmh()
+    if a.max_paths:
+        ctx.set_render_params(a.max_paths)
+    w, h = scene.width, scene.height
+    # warm-up: code objects and the largest path buffers
+    ctx.frame_create(w, h, 64)
+    ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, 64, scene.max_depth,
+                                                       scene.rr_start, 1))
+    ctx.synchronize()
+    spp, rows, hit = 1, [], None
+    while spp <= max_spp:
+        ctx.frame_create(w, h, spp)
+        t0 = time.perf_counter()
+        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, spp, scene.max_depth,
+                                                           scene.rr_start, 7))
+        img_a = ctx.frame_read()
+        dt = time.perf_counter() - t0
+        rays = ctx.rays_traced()
+        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, spp, scene.max_depth,
+                                                           scene.rr_start, 8))
+        img_b = ctx.frame_read()
+        err = two_seed_rel_rmse(img_a, img_b)
+        rows.append({"spp": spp, "s": round(dt, 4), "rel_rmse": round(err, 5),
+                     "mrays": round(rays / dt / 1e6, 1), "rays": rays})
+        log(f"{label}: {spp} spp {dt:.3f}s rel-RMSE {err:.4f} {rays / dt / 1e6:.0f} Mrays/s")
+        if err <= 0.01:
+            hit = rows[-1]
+            break
+        spp *= 2
+    params = ctx.get_render_params()
+    ctx.close()
+    return {"curve": rows, "to_1pct": hit, "render_params": params,
+            "error_estimate": "two-seed: ||I7 - I8|| / (sqrt2 ||(I7 + I8)/2||)"}
+
+
+def workload_render_c1(a) -> dict:
+    from aurora_amd import scenes
+
+    scene = scenes.cornell()  # 256^2, depth 4, RR from bounce 3
+    res = render_convergence(a, scene, "c1", 1 << 16)
+    cpu = None
+    if not a.no_cpu and res["to_1pct"]:
+        from tests import oracle_ffi
+
+        perm = oracle_ffi.nmh_build(scene.vertices)
+        tree = oracle_ffi.tree_arrays(scene, perm)
+        spp = 4
+        th = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        oracle_ffi.render(tree, scene, 1, spp, scene.max_depth, scene.rr_start, 7, threads=th)
+        dt = (time.perf_counter() - t0) * res["to_1pct"]["spp"] / spp
+        cpu = {"value": round(dt, 2), "unit": "s", "cores": th, "kind": "port",
+               "sample": f"oracle PBR render at {spp} spp, scaled to {res['to_1pct']['spp']} spp"}
+    hit = res["to_1pct"]
+    return {"metric": "s to 1% rel-RMSE (c1 Cornell, PBR)", "value": hit["s"] if hit else None,
+            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+            "data": "synthetic (seeded Cornell box + 2 icospheres)",
+            "config": {"workload": "c1: 256x256, depth 4, RR from bounce 3", **res},
+            "cpu_baseline": cpu}
+
+
+def workload_render_c2(a) -> dict:
+    from aurora_amd import scenes
+
+    scene = scenes.spheres()  # 1920x1080, depth 8
+    res = render_convergence(a, scene, "c2", 4096)
+    hit = res["to_1pct"]
+    return {"metric": "s to 1% rel-RMSE (c2 spheres 1080p, PBR)", "value": hit["s"] if hit else None,
+            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+            "data": "synthetic (seeded 200 icospheres + ground, 4 quad lights)",
+            "config": {"workload": "c2: 1920x1080, depth 8, RR from bounce 3", **res}}
+
+
+WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m,
+             "render_c1": workload_render_c1, "render_c2": workload_render_c2}
 
 
 def main() -> None:
@@ -233,6 +328,7 @@ def main() -> None:
     ap.add_argument("--workload", default="trace1m", choices=sorted(WORKLOADS))
     ap.add_argument("--tune", action="store_true", help="autotune trace launch params first")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")

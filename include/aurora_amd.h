@@ -149,9 +149,11 @@ aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int3
  * waves each own a contiguous ray range and refill idle lanes once `refill`
  * (1, 8, 16 or 32) lanes are idle; waves_per_cu = 0 uses the occupancy
  * limit.  persistent = 0: one lane per ray.  block: 64, 128 or 256.
+ * stack_lds_rows: how many of the deepest tree levels keep each lane's
+ * deferred ranges in LDS (0 = all); shallower levels use global memory.
  * Results do not depend on these values. */
 typedef struct aur_trace_params {
-  uint32_t persistent, block, refill, waves_per_cu;
+  uint32_t persistent, block, refill, waves_per_cu, stack_lds_rows;
 } aur_trace_params;
 aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* params);
 aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* params);
@@ -170,6 +172,17 @@ aur_status aur_frame_destroy(aur_context* ctx);
 /* Clears the accumulation buffer and renders opts->spp samples. */
 aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
                             const aur_render_opts* opts);
+/* Path-loop launch shape (no reference counterpart: the reference traced
+ * 65,536-ray tiles one sample at a time, src/render/monteCarloRaytracer.cpp:13).
+ * max_paths: paths in flight per wavefront pass; a pass traces
+ * clamp(max_paths / pixels, 1, remaining) samples of every pixel.  0 selects
+ * the default (2^21).  Images do not depend on it: samples are accumulated in
+ * sample order. */
+typedef struct aur_render_params {
+  uint32_t max_paths;
+} aur_render_params;
+aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* params);
+aur_status aur_get_render_params(aur_context* ctx, aur_render_params* params);
 /* Renders samples [sample_begin, sample_end) of opts->spp into the existing
  * accumulation (progressive rendering and resume; the counter RNG makes a
  * split render equal to an unsplit one). */

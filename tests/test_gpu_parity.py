@@ -167,3 +167,22 @@ def test_render_progressive_equals_full(gpu_ctx):
     gpu_ctx.frame_render_samples(s.camera, o, 3, 8)
     split = gpu_ctx.frame_read()
     assert np.array_equal(full, split)
+
+
+@pytest.mark.parametrize("max_paths", [1, 3 * 64 * 64, 0])
+def test_render_pass_batching_invariant(gpu_ctx, max_paths):
+    """Samples per pass (paths in flight) must not change a single bit."""
+    s = scenes.cornell(width=64, height=64)
+    _build(gpu_ctx, s)
+    gpu_ctx.frame_create(64, 64, 8)
+    o = native.Context.opts(native.MODE_PBR, 8, 4, 3, 5)
+    try:
+        gpu_ctx.set_render_params(1)  # one sample per pass
+        gpu_ctx.frame_render(s.camera, o)
+        one = gpu_ctx.frame_read()
+        gpu_ctx.set_render_params(max_paths)
+        gpu_ctx.frame_render(s.camera, o)
+        many = gpu_ctx.frame_read()
+    finally:
+        gpu_ctx.set_render_params(0)
+    assert np.array_equal(one, many)
