@@ -308,6 +308,26 @@ aur_status aur_frame_device_buffer(aur_context* ctx, float** rgba) {
   return AUR_OK;
 }
 
+aur_status aur_frame_copy(aur_context* ctx, float* rgba) {
+  if (!ctx || !rgba) return AUR_ERR_INVALID_ARG;
+  aur::FrameState& fr = ctx->frame;
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "frame: none");
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  AUR_HIP(ctx, hipMemcpyAsync(rgba, fr.pixels.p, sizeof(float4) * fr.width * fr.height,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return AUR_OK;
+}
+
 uint64_t aur_frame_rays_traced(const aur_context* ctx) { return ctx ? ctx->frame.rays_traced : 0; }
+
+aur_status aur_frame_adjoint(aur_context* ctx, const aur_camera* camera, const aur_render_opts* opts,
+                             const float* d_image, float* grad_albedo, float* grad_radiance) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (!camera || !opts) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: null argument");
+  if (opts->abi_version != AUR_ABI_VERSION) return ctx->fail(AUR_ERR_UNSUPPORTED, "adjoint: ABI version");
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  return aur::frame_adjoint(ctx, camera, opts, d_image, grad_albedo, grad_radiance);
+}
 
 }  // extern "C"

@@ -57,6 +57,12 @@ struct FrameState {
   DevBuf<uint32_t> counters;  // [0..1] queue lengths, [2] shadow count, [3..] stats
   DevBuf<float4> sh_o, sh_d, sh_c;  // shadow rays: origin+tmax, dir, contribution
   DevBuf<int32_t> sh_occ;
+  // adjoint replay: path-vertex records, per-block partial gradients,
+  // running gradient sums and the largest |dLoss/dI|
+  DevBuf<int32_t> rec_mat;
+  DevBuf<float> rec_rr, rec_w;
+  DevBuf<double> adj_partials, adj_total;
+  DevBuf<uint32_t> adj_absmax;
   size_t shadow_cap = 0;
   uint64_t rays_traced = 0;
 };
@@ -130,4 +136,6 @@ aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays
                                uint64_t* popped, uint64_t* passed);
 aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
                         uint32_t s0, uint32_t s1);
+aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
+                         const float* d_image, float* g_albedo, float* g_light);
 }  // namespace aur

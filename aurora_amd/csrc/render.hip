@@ -42,11 +42,14 @@ struct SceneDev {
   uint32_t num_lights;
 };
 
-__device__ inline const aur_shader& shader_of(const SceneDev& sc, uint32_t tri) {
+__device__ inline uint32_t shader_index(const SceneDev& sc, uint32_t tri) {
   uint32_t sid = sc.shader_ids[tri];
   sid = sid > 0 ? sid - 1 : 0;  // getSafeID (src/util/array.h:18)
-  if (sid >= sc.num_shaders) sid = 0;
-  return sc.shaders[sid];
+  return sid < sc.num_shaders ? sid : 0;
+}
+
+__device__ inline const aur_shader& shader_of(const SceneDev& sc, uint32_t tri) {
+  return sc.shaders[shader_index(sc, tri)];
 }
 
 __device__ inline V3 interp9(const float* base, uint32_t tri, float u, float v) {
@@ -311,6 +314,14 @@ struct PbrArgs {
   int32_t* sh_occ;
   uint32_t* counters;  // [0],[1] queue lengths
   unsigned long long* stats;
+  // path-vertex records of an adjoint replay (null in a plain render),
+  // indexed [bounce][path] and [bounce][slot][path]: shader index of the hit
+  // (-1 ends the path), Russian-roulette survival probability (1 when not
+  // applied) and each next-event slot's weight w, zero when occluded, with
+  // contribution = beta * albedo * light radiance * w.
+  int32_t* rec_mat;
+  float* rec_rr;
+  float* rec_w;
 };
 
 __device__ __forceinline__ uint32_t path_pixel(const PbrArgs& a, uint32_t q) { return q % a.npix; }
@@ -375,6 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, i
       [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
 }
 
+template <bool kRecord>
 __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
                             const uint32_t* __restrict__ qin, uint32_t* __restrict__ qout) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -383,11 +395,14 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
   const uint32_t p = qin[j];  // path index
   const uint32_t pix = path_pixel(a, p), smp = path_sample(a, p);
   const size_t base = (size_t)j * a.nslots;
+  const size_t np = (size_t)a.npix * a.batch;
   const aur_hit hit = a.hits[p];
   if (hit.id < 0) {
     for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s].w = 0.0f;
+    if (kRecord) a.rec_mat[bounce * np + p] = -1;
     return;
   }
+  if (kRecord) a.rec_mat[bounce * np + p] = (int32_t)shader_index(sc, (uint32_t)hit.id);
   const float4 o4 = a.ray_o[p], d4 = a.ray_d[p], b4 = a.beta[p];
   float4 L4 = a.radiance[p];
   const V3 O = mk(o4.x, o4.y, o4.z), D = mk(d4.x, d4.y, d4.z);
@@ -423,7 +438,7 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
           const V3 c = bf * (CI * wgt);
           const V3 so = P + (kEpsilon * wi);
           a.sh_o[base + e] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
-          a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+          a.sh_c[base + e] = make_float4(c.x, c.y, c.z, wgt * kInvPi);
           flag = 1.0f;
         }
         a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
@@ -437,7 +452,7 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
       if (cosS > 0.0f) {
         const V3 c = bf * ((CI / (dist * dist)) * cosS);
         a.sh_o[base + e] = make_float4(P.x, P.y, P.z, dist);
-        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, cosS / (dist * dist) * kInvPi);
         flag = 1.0f;
       }
       a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
@@ -449,24 +464,29 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
       if (cosS > 0.0f) {
         const V3 c = bf * (CI * cosS);
         a.sh_o[base + e] = make_float4(P.x, P.y, P.z, kInfinity);
-        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, cosS * kInvPi);
         flag = 1.0f;
       }
       a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
     } else {
       const V3 c = bf * (CI * Lt.ambient);
-      a.sh_c[base + e] = make_float4(c.x, c.y, c.z, 0.0f);
+      a.sh_c[base + e] = make_float4(c.x, c.y, c.z, Lt.ambient * kInvPi);
       a.sh_d[base + e] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);  // no shadow ray
       ++e;
     }
   }
   if (bounce + 1 >= a.max_depth) return;
   const U4 r = pbr_draw(a.seed, pix, smp, bounce, 1);
+  if (kRecord) a.rec_rr[bounce * np + p] = 1.0f;
   if (bounce >= a.rr_start) {
     const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
-    if (u01(r.z) >= q) return;
+    if (u01(r.z) >= q) {
+      if (kRecord) a.rec_mat[(bounce + 1) * np + p] = -1;
+      return;
+    }
     beta = beta / q;
+    if (kRecord) a.rec_rr[bounce * np + p] = q;
   }
   float dx, dy;
   concentric_disk(u01(r.x), u01(r.y), dx, dy);
@@ -510,20 +530,25 @@ __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, SceneDev sc, i
   count_add(&a.stats[1], traced);
 }
 
-__global__ void k_pbr_resolve(PbrArgs a, int cin, const uint32_t* __restrict__ qin) {
+template <bool kRecord>
+__global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_t* __restrict__ qin) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.counters[cin]) return;
   const uint32_t p = qin[j];
   if (a.hits[p].id < 0) return;
   const size_t base = (size_t)j * a.nslots;
+  const size_t np = (size_t)a.npix * a.batch;
   float4 L4 = a.radiance[p];
   V3 L = mk(L4.x, L4.y, L4.z);
   for (uint32_t s = 0; s < a.nslots; ++s) {
     const float flag = a.sh_d[base + s].w;
-    if (flag == 2.0f || (flag == 1.0f && a.sh_occ[base + s] == 0)) {
-      const float4 c = a.sh_c[base + s];
+    const bool lit = flag == 2.0f || (flag == 1.0f && a.sh_occ[base + s] == 0);
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lit) {
+      c = a.sh_c[base + s];
       L = L + mk(c.x, c.y, c.z);
     }
+    if (kRecord) a.rec_w[((size_t)bounce * a.nslots + s) * np + p] = c.w;
   }
   a.radiance[p] = make_float4(L.x, L.y, L.z, L4.w);
 }
@@ -547,6 +572,140 @@ __global__ void k_pbr_accumulate(PbrArgs a, float4* __restrict__ pixels) {
   pixels[p] = px;
 }
 
+// ---------------------------------------------------------------- adjoint --
+//
+// Gradient of a scalar image loss with respect to each shader's albedo
+// (diffuse_color * diffuse) and each light's radiance (color * intensity),
+// given dLoss/dI per pixel.  For one path with vertices b = 0..n-1 (shader
+// m_b, albedo rho_b, emission e_b, Russian-roulette probability q_b, and
+// next-event weight W_{b,l} per light l), the estimate is
+//   sum_b beta_b (e_b + rho_b D_b),   D_b = sum_l CI_l W_{b,l},
+//   beta_0 = 1,  beta_{b+1} = beta_b rho_b / q_b
+// (cosine-weighted BSDF sampling makes the continuation weight rho_b).  With
+// the suffix G_b = e_b + rho_b R_b, R_b = D_b + G_{b+1} / q_b, the
+// derivatives are d/d rho_b = beta_b R_b and d/d CI_l = sum_b beta_b rho_b
+// W_{b,l}; the RR probabilities are held fixed.  The sweep visits vertices
+// forward for beta and the light terms, then backward for R.
+
+// Per-block LDS accumulation is exact integer arithmetic on contributions
+// quantised to 2^-28 of the largest per-sample pixel adjoint, so the sum does
+// not depend on the order in which lanes add.
+constexpr float kAdjQuant = 268435456.0f;  // 2^28
+
+__global__ void k_adj_absmax(const float4* __restrict__ dimg, uint32_t npix, uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    const float4 g = dimg[p];
+    const float a = fmaxf(fabsf(g.x), fmaxf(fabsf(g.y), fabsf(g.z)));
+    m = max(m, __float_as_uint(a));  // non-negative floats order as integers
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+  if ((threadIdx.x & 63u) == 0 && m) atomicMax(out, m);
+}
+
+__device__ inline void adj_add(unsigned long long* acc, uint32_t i, float v, float scale) {
+  if (v == 0.0f) return;
+  const float x = fminf(fmaxf(v * scale, -4.0e18f), 4.0e18f);
+  atomicAdd(acc + i, (unsigned long long)__float2ll_rn(x));
+}
+
+template <int kMax>
+__global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, SceneDev sc, const float4* __restrict__ dimg,
+                                                   const uint32_t* __restrict__ absmax,
+                                                   double* __restrict__ partials, uint32_t nparams) {
+  extern __shared__ unsigned long long acc[];
+  for (uint32_t i = threadIdx.x; i < nparams; i += blockDim.x) acc[i] = 0ull;
+  __syncthreads();
+  const float dmax = __uint_as_float(*absmax);
+  const float scale = dmax > 0.0f ? kAdjQuant * (float)a.spp / dmax : 0.0f;
+  const float inv_spp = 1.0f / (float)a.spp;
+  const uint32_t np = a.npix * a.batch;
+  const uint32_t lbase = 3u * sc.num_shaders;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; scale > 0.0f && q < np;
+       q += gridDim.x * blockDim.x) {
+    const float4 g = dimg[path_pixel(a, q)];
+    const V3 dl = mk(g.x * inv_spp, g.y * inv_spp, g.z * inv_spp);
+    if (dl.x == 0.0f && dl.y == 0.0f && dl.z == 0.0f) continue;
+    V3 beta = mk(1.0f, 1.0f, 1.0f);
+    V3 betas[kMax], D[kMax];
+    int32_t mats[kMax];
+    float rr[kMax];
+    uint32_t n = 0;
+#pragma unroll
+    for (int b = 0; b < kMax; ++b) {
+      if ((uint32_t)b < a.max_depth && n == (uint32_t)b) {
+        const int32_t m = a.rec_mat[(size_t)b * np + q];
+        if (m >= 0) {
+          const aur_shader& sh = sc.shaders[m];
+          const V3 rho = ld3(sh.diffuse_color) * sh.diffuse;
+          V3 d = mk(0.0f, 0.0f, 0.0f);
+          uint32_t s = 0;
+          for (uint32_t li = 0; li < sc.num_lights; ++li) {
+            const aur_light& Lt = sc.lights[li];
+            const uint32_t ns = Lt.type == AUR_LIGHT_AREA ? (uint32_t)max(1, (int)Lt.samples) : 1u;
+            float W = 0.0f;
+            for (uint32_t k = 0; k < ns; ++k, ++s) W += a.rec_w[((size_t)b * a.nslots + s) * np + q];
+            if (W != 0.0f) {
+              const V3 CI = ld3(Lt.color) * Lt.intensity;
+              d = d + CI * W;
+              const V3 gl = dl * beta * rho * W;
+              adj_add(acc, lbase + 3 * li + 0, gl.x, scale);
+              adj_add(acc, lbase + 3 * li + 1, gl.y, scale);
+              adj_add(acc, lbase + 3 * li + 2, gl.z, scale);
+            }
+          }
+          betas[b] = beta;
+          D[b] = d;
+          mats[b] = m;
+          const bool more = (uint32_t)b + 1 < a.max_depth;
+          rr[b] = more ? a.rec_rr[(size_t)b * np + q] : 1.0f;
+          beta = (beta / rr[b]) * rho;
+          n = b + 1;
+        }
+      }
+    }
+    V3 G = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int b = kMax - 1; b >= 0; --b) {
+      if ((uint32_t)b < n) {
+        const aur_shader& sh = sc.shaders[mats[b]];
+        const V3 rho = ld3(sh.diffuse_color) * sh.diffuse;
+        const V3 R = D[b] + G / rr[b];
+        const V3 ga = dl * betas[b] * R;
+        adj_add(acc, 3 * mats[b] + 0, ga.x, scale);
+        adj_add(acc, 3 * mats[b] + 1, ga.y, scale);
+        adj_add(acc, 3 * mats[b] + 2, ga.z, scale);
+        G = ld3(sh.emission_color) + rho * R;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nparams; i += blockDim.x)
+    partials[(size_t)blockIdx.x * nparams + i] = scale > 0.0f ? (double)(long long)acc[i] / (double)scale : 0.0;
+}
+
+// total[i] += sum over blocks of partials[block][i], blocks in order.
+__global__ void k_adj_reduce(const double* __restrict__ partials, uint32_t nblocks, uint32_t nparams,
+                             double* __restrict__ total) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nparams) return;
+  double s = 0.0;
+  for (uint32_t b = 0; b < nblocks; ++b) s += partials[(size_t)b * nparams + i];
+  total[i] += s;
+}
+
+__global__ void k_adj_finish(const double* __restrict__ total, uint32_t nshader3, uint32_t nlight3,
+                             float* __restrict__ g_albedo, float* __restrict__ g_light) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nshader3) g_albedo[i] = (float)total[i];
+  else if (i < nshader3 + nlight3 && g_light) g_light[i - nshader3] = (float)total[i];
+}
+
+__global__ void k_zero_f64(double* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -564,6 +723,110 @@ uint32_t resident_grid(aur_context* ctx, const void* fn, size_t lds, uint64_t ma
   const uint64_t need = (max_items + kBlock - 1) / kBlock;
   if (blocks > need) blocks = need;
   return (uint32_t)(blocks ? blocks : 1);
+}
+
+__global__ void k_adj_zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+constexpr uint32_t kAdjMaxDepth = 16;
+constexpr uint32_t kAdjMaxParams = 6144;  // 48 KiB of LDS accumulators
+
+struct AdjointState {
+  const float4* dimg = nullptr;  // dLoss/dI per pixel (RGB used)
+  uint32_t* absmax = nullptr;    // max |dLoss/dI| bits
+  double* total = nullptr;       // running gradient sums
+  uint32_t nparams = 0;
+};
+
+// Sample passes [s0, s1) of the PBR path loop.  Without `adj` each pass is
+// accumulated into the frame; with it the passes are replayed with vertex
+// records and each is followed by the adjoint sweep and the ordered reduction
+// of the per-block partial gradients.
+aur_status pbr_passes(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o, uint32_t s0,
+                      uint32_t s1, unsigned long long* stats, AdjointState* adj) {
+  FrameState& fr = ctx->frame;
+  hipStream_t s = ctx->stream;
+  const uint32_t w = fr.width, h = fr.height, npix = w * h;
+  SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
+              ctx->num_shaders, ctx->lights.p, ctx->num_lights};
+  const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
+  const uint32_t nslots = ctx->num_slots;
+  // samples per pass: enough paths in flight to fill the device
+  const uint32_t max_paths = std::max<uint32_t>(ctx->render_params.max_paths, 1u);
+  const uint32_t batch_cap = std::max<uint32_t>(1u, max_paths / npix);
+  const uint32_t batch = std::min<uint32_t>(batch_cap, std::max<uint32_t>(1u, s1 - s0));
+  const uint64_t npaths = (uint64_t)npix * batch;
+  const uint64_t rec_words = adj ? npaths * o->max_depth * std::max<uint32_t>(nslots, 1u) : 0;
+  if (npaths * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull || rec_words > (1ull << 34))
+    return ctx->fail(AUR_ERR_INVALID_ARG, "render: frame too large");
+  const size_t nsh = std::max<size_t>(1, (size_t)npaths * nslots);
+  AUR_HIP(ctx, fr.ray_o.reserve(npaths));
+  AUR_HIP(ctx, fr.ray_d.reserve(npaths));
+  AUR_HIP(ctx, fr.beta.reserve(npaths));
+  AUR_HIP(ctx, fr.radiance.reserve(npaths));
+  AUR_HIP(ctx, fr.hits.reserve(npaths));
+  AUR_HIP(ctx, fr.queue[0].reserve(npaths));
+  AUR_HIP(ctx, fr.queue[1].reserve(npaths));
+  AUR_HIP(ctx, fr.sh_o.reserve(nsh));
+  AUR_HIP(ctx, fr.sh_d.reserve(nsh));
+  AUR_HIP(ctx, fr.sh_c.reserve(nsh));
+  AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
+  PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots, npix, batch,
+            fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
+            fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats,
+            nullptr, nullptr, nullptr};
+  uint32_t sweep_grid = 0;
+  if (adj) {
+    AUR_HIP(ctx, fr.rec_mat.reserve(npaths * o->max_depth));
+    AUR_HIP(ctx, fr.rec_rr.reserve(npaths * o->max_depth));
+    AUR_HIP(ctx, fr.rec_w.reserve(std::max<uint64_t>(1, npaths * o->max_depth * nslots)));
+    a.rec_mat = fr.rec_mat.p;
+    a.rec_rr = fr.rec_rr.p;
+    a.rec_w = fr.rec_w.p;
+    // a few resident blocks per CU; each keeps LDS accumulators for all parameters
+    sweep_grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4, (npaths + 255) / 256);
+    AUR_HIP(ctx, fr.adj_partials.reserve((size_t)sweep_grid * adj->nparams));
+  }
+  const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npaths);
+  const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, npaths * nslots);
+  for (uint32_t smp = s0; smp < s1; smp += a.batch) {
+    a.sample = smp;
+    a.batch = std::min<uint32_t>(batch, s1 - smp);
+    const uint64_t np = (uint64_t)npix * a.batch;
+    k_pbr_camera<<<nblocks(np, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
+    int cin = 0;
+    for (uint32_t b = 0; b < o->max_depth; ++b) {
+      uint32_t* qin = fr.queue[cin].p;
+      uint32_t* qout = fr.queue[cin ^ 1].p;
+      k_pbr_extend<<<grid_extend, kBlock, lds, s>>>(a, sc, cin, qin);
+      if (adj)
+        k_pbr_shade<true><<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
+      else
+        k_pbr_shade<false><<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
+      if (nslots) {
+        k_pbr_shadow<<<grid_shadow, kBlock, lds, s>>>(a, sc, cin);
+        if (adj)
+          k_pbr_resolve<true><<<nblocks(np, 256), 256, 0, s>>>(a, cin, b, qin);
+        else
+          k_pbr_resolve<false><<<nblocks(np, 256), 256, 0, s>>>(a, cin, b, qin);
+      }
+      cin ^= 1;
+    }
+    if (!adj) {
+      k_pbr_accumulate<<<nblocks(npix, 256), 256, 0, s>>>(a, fr.pixels.p);
+    } else {
+      const size_t slds = (size_t)adj->nparams * sizeof(unsigned long long);
+      const uint32_t g = (uint32_t)std::min<uint64_t>(sweep_grid, (np + 255) / 256);
+      if (o->max_depth <= 8)
+        k_adj_sweep<8><<<g, 256, slds, s>>>(a, sc, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
+      else
+        k_adj_sweep<16><<<g, 256, slds, s>>>(a, sc, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
+      k_adj_reduce<<<nblocks(adj->nparams, 64), 64, 0, s>>>(fr.adj_partials.p, g, adj->nparams, adj->total);
+    }
+  }
+  AUR_LAUNCH_CHECK(ctx, "pbr kernels");
+  return AUR_OK;
 }
 
 }  // namespace
@@ -592,52 +855,50 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
     }
     AUR_LAUNCH_CHECK(ctx, "k_compat");
   } else {
-    const uint32_t nslots = ctx->num_slots;
-    // samples per pass: enough paths in flight to fill the device
-    const uint32_t max_paths = std::max<uint32_t>(ctx->render_params.max_paths, 1u);
-    const uint32_t batch_cap = std::max<uint32_t>(1u, max_paths / npix);
-    const uint32_t batch = std::min<uint32_t>(batch_cap, std::max<uint32_t>(1u, s1 - s0));
-    const uint64_t npaths = (uint64_t)npix * batch;
-    if (npaths * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull)
-      return ctx->fail(AUR_ERR_INVALID_ARG, "render: frame too large");
-    const size_t nsh = std::max<size_t>(1, (size_t)npaths * nslots);
-    AUR_HIP(ctx, fr.ray_o.reserve(npaths));
-    AUR_HIP(ctx, fr.ray_d.reserve(npaths));
-    AUR_HIP(ctx, fr.beta.reserve(npaths));
-    AUR_HIP(ctx, fr.radiance.reserve(npaths));
-    AUR_HIP(ctx, fr.hits.reserve(npaths));
-    AUR_HIP(ctx, fr.queue[0].reserve(npaths));
-    AUR_HIP(ctx, fr.queue[1].reserve(npaths));
-    AUR_HIP(ctx, fr.sh_o.reserve(nsh));
-    AUR_HIP(ctx, fr.sh_d.reserve(nsh));
-    AUR_HIP(ctx, fr.sh_c.reserve(nsh));
-    AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
-    PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots, npix, batch,
-              fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
-              fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats};
-    const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npaths);
-    const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, npaths * nslots);
-    for (uint32_t smp = s0; smp < s1; smp += a.batch) {
-      a.sample = smp;
-      a.batch = std::min<uint32_t>(batch, s1 - smp);
-      const uint64_t np = (uint64_t)npix * a.batch;
-      k_pbr_camera<<<nblocks(np, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
-      int cin = 0;
-      for (uint32_t b = 0; b < o->max_depth; ++b) {
-        uint32_t* qin = fr.queue[cin].p;
-        uint32_t* qout = fr.queue[cin ^ 1].p;
-        k_pbr_extend<<<grid_extend, kBlock, lds, s>>>(a, sc, cin, qin);
-        k_pbr_shade<<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
-        if (nslots) {
-          k_pbr_shadow<<<grid_shadow, kBlock, lds, s>>>(a, sc, cin);
-          k_pbr_resolve<<<nblocks(np, 256), 256, 0, s>>>(a, cin, qin);
-        }
-        cin ^= 1;
-      }
-      k_pbr_accumulate<<<nblocks(npix, 256), 256, 0, s>>>(a, fr.pixels.p);
-    }
-    AUR_LAUNCH_CHECK(ctx, "pbr kernels");
+    const aur_status st = pbr_passes(ctx, cam, o, s0, s1, stats, nullptr);
+    if (st != AUR_OK) return st;
   }
+  unsigned long long hs[2] = {0, 0};
+  AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+  AUR_HIP(ctx, hipStreamSynchronize(s));
+  fr.rays_traced = hs[0] + hs[1];
+  return AUR_OK;
+}
+
+aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o,
+                         const float* d_image, float* g_albedo, float* g_light) {
+  FrameState& fr = ctx->frame;
+  if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "adjoint: geometry not built");
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "adjoint: no frame");
+  if (ctx->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "adjoint: no shaders uploaded");
+  if (o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: PBR mode only");
+  if (o->spp == 0 || o->max_depth == 0) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: spp and max_depth must be > 0");
+  if (o->max_depth > kAdjMaxDepth) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: max_depth > 16");
+  if (!d_image || !g_albedo || (ctx->num_lights && !g_light))
+    return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: null buffer");
+  const uint32_t nparams = 3u * (ctx->num_shaders + ctx->num_lights);
+  if (nparams > kAdjMaxParams) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: more than 2048 shaders + lights");
+  hipStream_t s = ctx->stream;
+  const uint32_t npix = fr.width * fr.height;
+  AUR_HIP(ctx, fr.counters.reserve(8));
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
+  k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
+  AdjointState adj;
+  adj.dimg = reinterpret_cast<const float4*>(d_image);
+  adj.nparams = nparams;
+  AUR_HIP(ctx, fr.adj_total.reserve(nparams));
+  AUR_HIP(ctx, fr.adj_absmax.reserve(1));
+  adj.total = fr.adj_total.p;
+  adj.absmax = fr.adj_absmax.p;
+  k_zero_f64<<<nblocks(nparams, 256), 256, 0, s>>>(adj.total, nparams);
+  k_adj_zero_u32<<<1, 64, 0, s>>>(adj.absmax);
+  k_adj_absmax<<<std::min<uint32_t>(nblocks(npix, 256), 1024), 256, 0, s>>>(adj.dimg, npix, adj.absmax);
+  AUR_LAUNCH_CHECK(ctx, "adjoint setup");
+  const aur_status st = pbr_passes(ctx, cam, o, 0, o->spp, stats, &adj);
+  if (st != AUR_OK) return st;
+  k_adj_finish<<<nblocks(nparams, 256), 256, 0, s>>>(adj.total, 3u * ctx->num_shaders,
+                                                     3u * ctx->num_lights, g_albedo, g_light);
+  AUR_LAUNCH_CHECK(ctx, "k_adj_finish");
   unsigned long long hs[2] = {0, 0};
   AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
   AUR_HIP(ctx, hipStreamSynchronize(s));

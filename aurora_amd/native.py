@@ -107,6 +107,8 @@ def load():
         "aur_frame_read": ([P, P], i32),
         "aur_frame_device_buffer": ([P, ctypes.POINTER(P)], i32),
         "aur_frame_rays_traced": ([P], u64),
+        "aur_frame_copy": ([P, P], i32),
+        "aur_frame_adjoint": ([P, P, ctypes.POINTER(RenderOpts), P, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -310,3 +312,17 @@ class Context:
 
     def rays_traced(self) -> int:
         return int(self._lib.aur_frame_rays_traced(self._h))
+
+    def frame_copy_ptr(self, rgba_ptr: int) -> None:
+        """Copies the float4 accumulation buffer to device memory at rgba_ptr."""
+        self._check(self._lib.aur_frame_copy(self._h, ctypes.c_void_p(rgba_ptr)), "frame_copy")
+
+    def frame_adjoint_ptr(self, camera: np.ndarray, opts: RenderOpts, d_image_ptr: int,
+                          grad_albedo_ptr: int, grad_radiance_ptr: int) -> None:
+        """Device pointers: dLoss/dI float4 per pixel in, gradients out."""
+        cam = np.ascontiguousarray(camera, dtype=scenes.CAMERA_DTYPE)
+        self._check(self._lib.aur_frame_adjoint(self._h, _ptr(cam), ctypes.byref(opts),
+                                                ctypes.c_void_p(d_image_ptr),
+                                                ctypes.c_void_p(grad_albedo_ptr),
+                                                ctypes.c_void_p(grad_radiance_ptr or None)),
+                    "frame_adjoint")

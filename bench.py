@@ -251,7 +251,9 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
     ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, 64, scene.max_depth,
                                                        scene.rr_start, 1))
     ctx.synchronize()
-    spp, rows, hit = 1, [], None
+    spp, rows, hit = (a.spp or 1), [], None
+    if a.spp:
+        max_spp = a.spp  # one render at --spp (profiling)
     while spp <= max_spp:
         ctx.frame_create(w, h, spp)
         t0 = time.perf_counter()
@@ -329,6 +331,7 @@ def main() -> None:
     ap.add_argument("--tune", action="store_true", help="autotune trace launch params first")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
+    ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")

@@ -193,11 +193,33 @@ aur_status aur_frame_clear(aur_context* ctx);
 /* Host float RGBA, width*height*4 floats, row 0 = bottom (framebuffer(),
  * src/render/monteCarloRaytracer.cpp:122-127). */
 aur_status aur_frame_read(aur_context* ctx, float* rgba);
+/* Copies the float4 accumulation buffer (width*height*4 floats, same layout
+ * as aur_frame_read) to device memory at rgba; synchronous. */
+aur_status aur_frame_copy(aur_context* ctx, float* rgba);
 /* Device pointer of the float4 accumulation buffer. */
 aur_status aur_frame_device_buffer(aur_context* ctx, float** rgba);
 /* Number of ray segments (camera + extension + shadow) traced by the last
  * aur_frame_render / aur_frame_render_samples call. */
 uint64_t aur_frame_rays_traced(const aur_context* ctx);
+
+/* ---- adjoint (no reference counterpart: the reference renders only; this
+ * is the differentiable form of the AUR_MODE_PBR path loop) ----
+ * Gradient of a scalar loss of the frame image with respect to each shader's
+ * albedo (diffuse_color * diffuse, RGB) and each light's radiance
+ * (color * intensity, RGB), for the image aur_frame_render produces with the
+ * same camera, opts, geometry, shaders and lights.  The paths are replayed
+ * (same counter-based random numbers), recording each vertex's shader,
+ * Russian-roulette probability and next-event weights in device memory for
+ * one pass of paths at a time, then swept backwards.  Russian-roulette
+ * probabilities are treated as constants.  Device pointers, queued on the
+ * context stream and synchronised before return:
+ *   d_image       float4 per pixel, dLoss/dI in the framebuffer layout (A unused)
+ *   grad_albedo   float[3 * shader count], overwritten
+ *   grad_radiance float[3 * light count], overwritten (may be NULL without lights)
+ * Requires mode AUR_MODE_PBR, max_depth <= 16, shaders + lights <= 2048.
+ * Deterministic: the same inputs give the same bits. */
+aur_status aur_frame_adjoint(aur_context* ctx, const aur_camera* camera, const aur_render_opts* opts,
+                             const float* d_image, float* grad_albedo, float* grad_radiance);
 
 #ifdef __cplusplus
 }

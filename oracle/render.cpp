@@ -26,9 +26,13 @@
 // counter-based RNG.  The same float operations run on the GPU.
 //
 // cudaGenerateTB (utility.cu:136-180) is restated as orc_generate_tb.
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "oracle_math.h"
 
@@ -386,7 +390,23 @@ inline void onb(f3 n, f3& t1, f3& t2) {
   t2 = mk3(b, sign + n.y * n.y * a, -n.y);
 }
 
-Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pixel, uint32_t sample) {
+// One path vertex as the adjoint needs it: shader index, Russian-roulette
+// probability applied after it (1 when none), and per light the unoccluded
+// next-event weight W with contribution beta * albedo * radiance * W / pi.
+struct Vertex {
+  uint32_t shader;
+  float q;
+  std::vector<float> W;
+};
+
+inline uint32_t shader_index(const Scene& sc, uint32_t tri) {
+  uint32_t sid = sc.shaderIds[tri];
+  sid = sid > 0 ? sid - 1 : 0;
+  return sid < sc.nShaders ? sid : 0;
+}
+
+Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pixel, uint32_t sample,
+                 std::vector<Vertex>* rec = nullptr) {
   f3 beta = mk3(1.0f), L = mk3(0.0f);
   bool covered = false;
   Ray ray{origin, dir, kInfinity, 1.0f, 1.0f};
@@ -401,6 +421,7 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
     const Shader& sh = shader_of(sc, (uint32_t)id);
     f3 N = normalize(interp(sc.normals, id, hu, hv));
     if (dot(N, wo) < 0.0f) N = -N;
+    if (rec) rec->push_back(Vertex{shader_index(sc, (uint32_t)id), 1.0f, std::vector<float>(sc.nLights, 0.0f)});
     const f3 albedo = v3(sh.diffuseColor) * sh.diffuse;
     const f3 f = albedo * kInvPi;
     L = L + beta * v3(sh.emissionColor);
@@ -425,6 +446,7 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
             if (!occluded(sc, sr)) {
               const float w = (cosS * cosL * Lt.area) / (dist * dist) * inv;
               L = L + bf * (CI * w);
+              if (rec) rec->back().W[li] += w;
             }
           }
         }
@@ -435,17 +457,24 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
         const float cosS = dot(N, wi);
         if (cosS > 0.0f) {
           Ray sr{P, wi, dist, 1.0f, 1.0f};
-          if (!occluded(sc, sr)) L = L + bf * ((CI / (dist * dist)) * cosS);
+          if (!occluded(sc, sr)) {
+            L = L + bf * ((CI / (dist * dist)) * cosS);
+            if (rec) rec->back().W[li] += cosS / (dist * dist);
+          }
         }
       } else if (Lt.type == 2) {
         const f3 wi = -v3(Lt.direction);
         const float cosS = dot(N, wi);
         if (cosS > 0.0f) {
           Ray sr{P, wi, kInfinity, 1.0f, 1.0f};
-          if (!occluded(sc, sr)) L = L + bf * (CI * cosS);
+          if (!occluded(sc, sr)) {
+            L = L + bf * (CI * cosS);
+            if (rec) rec->back().W[li] += cosS;
+          }
         }
       } else {
         L = L + bf * (CI * Lt.ambient);
+        if (rec) rec->back().W[li] += Lt.ambient;
       }
     }
     if (b + 1 >= o.max_depth) break;
@@ -454,6 +483,7 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
       const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
       if (u01(r.z) >= q) break;
       beta = beta / q;
+      if (rec) rec->back().q = q;
     }
     float dx, dy;
     concentric_disk(u01(r.x), u01(r.y), dx, dy);
@@ -562,6 +592,90 @@ int orc_render(uint32_t count, const float* verts, const float* normals, const f
       }
     }
     std::memcpy(rgba + size_t(q) * 4, acc, sizeof(acc));
+  }
+  return 0;
+}
+
+// Gradient of sum_p <dimg_p, I_p> (dimg = dLoss/dI, float4 per pixel, RGB
+// used) for the PBR image of orc_render, with respect to each shader's
+// albedo (diffuseColor * diffuse) and each light's radiance (color *
+// intensity), in double.  Product-rule form: every estimator term is a
+// product of the albedos of the vertices before it (and of its own vertex
+// for a next-event term) with constant factors, so its derivative with
+// respect to albedo m is term * (times m occurs) / albedo_m; Russian-roulette
+// probabilities are constants.  Requires albedo channels > 0.
+int orc_adjoint(uint32_t count, const float* verts, const float* normals, const float* tangents,
+                const float* bitangents, const uint16_t* shaderIds, uint32_t nShaders,
+                const void* shaders, uint32_t nLights, const void* lights, const void* camera,
+                uint32_t w, uint32_t h, const uint32_t* opts, const float* dimg, double* g_albedo,
+                double* g_light, int threads) {
+  if (count < 2 || (count & 1u) || nShaders == 0 || w == 0 || h == 0) return -1;
+  Scene sc{count, verts, normals, tangents, bitangents, shaderIds, nShaders,
+           (const Shader*)shaders, nLights, (const Light*)lights};
+  const Opts o{opts[0], opts[1], opts[2], opts[3], opts[4]};
+  if (o.spp == 0 || o.mode != 1) return -1;
+  const Camera& cam = *(const Camera*)camera;
+  const int nt = threads > 0 ? threads : 1;
+  std::vector<std::vector<double>> ga(nt, std::vector<double>(size_t(nShaders) * 3, 0.0));
+  std::vector<std::vector<double>> gl(nt, std::vector<double>(size_t(nLights) * 3, 0.0));
+  auto rho_of = [&](uint32_t m, int c) {
+    const Shader& sh = sc.shaders[m];
+    return (double)sh.diffuseColor[c] * (double)sh.diffuse;
+  };
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+  for (int64_t pix = 0; pix < (int64_t)w * h; ++pix) {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    const uint32_t x = (uint32_t)pix % w, y = (uint32_t)pix / w;
+    std::vector<Vertex> path;
+    for (uint32_t s = 0; s < o.spp; ++s) {
+      path.clear();
+      const f3 d = camera_dir_pbr(cam, w, h, x, y, o.spp, o.seed, (uint32_t)pix, s);
+      shade_pbr(sc, o, v3(cam.position), d, (uint32_t)pix, s, &path);
+      for (int c = 0; c < 3; ++c) {
+        const double dl = (double)dimg[pix * 4 + c] / (double)o.spp;
+        if (dl == 0.0) continue;
+        double beta = 1.0;
+        for (size_t b = 0; b < path.size(); ++b) {
+          const Vertex& v = path[b];
+          const double rho = rho_of(v.shader, c);
+          const Shader& sh = sc.shaders[v.shader];
+          // emission term: albedos of vertices 0..b-1
+          const double te = beta * (double)sh.emissionColor[c];
+          // next-event terms: albedos of vertices 0..b
+          double tn = 0.0;
+          for (uint32_t l = 0; l < nLights; ++l) {
+            if (v.W[l] == 0.0f) continue;
+            const Light& Lt = sc.lights[l];
+            const double ci = (double)Lt.color[c] * (double)Lt.intensity;
+            const double k = beta * rho * (double)v.W[l] / M_PI;
+            tn += k * ci;
+            gl[t][size_t(l) * 3 + c] += dl * k;
+          }
+          for (size_t i = 0; i <= b; ++i) {
+            const uint32_t m = path[i].shader;
+            const double r = rho_of(m, c);
+            double term = tn;
+            if (i < b) term += te;
+            ga[t][size_t(m) * 3 + c] += dl * term / r;
+          }
+          beta = beta * rho / (double)v.q;
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < size_t(nShaders) * 3; ++i) {
+    double acc = 0.0;
+    for (int t = 0; t < nt; ++t) acc += ga[t][i];
+    g_albedo[i] = acc;
+  }
+  for (size_t i = 0; i < size_t(nLights) * 3; ++i) {
+    double acc = 0.0;
+    for (int t = 0; t < nt; ++t) acc += gl[t][i];
+    g_light[i] = acc;
   }
   return 0;
 }

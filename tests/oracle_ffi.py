@@ -45,6 +45,8 @@ def lib():
         L.orc_camera_ray.argtypes = [P, u32, u32, u32, u32, u32, u32, P]
         L.orc_render.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, u32, P, P, u32, P, i32]
         L.orc_render.restype = i32
+        L.orc_adjoint.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, u32, P, P, P, P, i32]
+        L.orc_adjoint.restype = i32
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_philox.argtypes = [P, u32, u32, P]
         _lib = L
@@ -144,6 +146,34 @@ def render(tree, scene, mode: int, spp: int, max_depth: int, rr_start: int, seed
     if rc != 0:
         raise RuntimeError(f"orc_render failed: {rc}")
     return out
+
+
+def adjoint(tree, scene, spp: int, max_depth: int, rr_start: int, seed: int, dimg: np.ndarray,
+            threads: int = 0):
+    """Gradients (albedo (n_shaders, 3), radiance (n_lights, 3)), float64, of
+    sum <dimg, I> for the PBR image; dimg float32 (npix, 4)."""
+    opts = np.array([1, spp, max_depth, rr_start, seed], dtype=np.uint32)
+    w, h = scene.width, scene.height
+    d = np.ascontiguousarray(dimg, np.float32).reshape(w * h, 4)
+    v = np.ascontiguousarray(tree["vertices"], np.float32)
+    nrm = np.ascontiguousarray(tree["normals"], np.float32)
+    T = np.ascontiguousarray(tree["tangents"], np.float32)
+    B = np.ascontiguousarray(tree["bitangents"], np.float32)
+    ids = np.ascontiguousarray(tree["shader_ids"], np.uint16)
+    sh = np.ascontiguousarray(scene.shaders)
+    li = np.ascontiguousarray(scene.lights)
+    cam = np.ascontiguousarray(scene.camera)
+    ga = np.zeros((sh.shape[0], 3), np.float64)
+    gl = np.zeros((max(li.shape[0], 1), 3), np.float64)
+    th = threads or os.cpu_count() or 1
+    rc = lib().orc_adjoint(v.shape[0], _p(v), _p(nrm), _p(T), _p(B), _p(ids), sh.shape[0],
+                           sh.ctypes.data_as(ctypes.c_void_p), li.shape[0],
+                           li.ctypes.data_as(ctypes.c_void_p) if li.shape[0] else None,
+                           cam.ctypes.data_as(ctypes.c_void_p), w, h, _p(opts), _p(d), _p(ga),
+                           _p(gl), th)
+    if rc != 0:
+        raise RuntimeError(f"orc_adjoint failed: {rc}")
+    return ga, gl[: li.shape[0]]
 
 
 def tree_arrays(scene, perm: np.ndarray) -> dict:
