@@ -12,6 +12,12 @@ backward: aur_frame_adjoint with dLoss/dI as a float4 device buffer.
 With a fixed seed the renders use common random numbers, so the image is a
 smooth function of the parameters when Russian roulette is off (rr_start >=
 max_depth) and the gradient is exactly the derivative of the estimate.
+
+decorrelate=True replays the backward pass with an independent seed: for a
+loss such as ||I - T||^2 the product of dLoss/dI (from the forward samples)
+and dI/dtheta (from independent samples) is then an unbiased estimate of the
+gradient of ||E[I] - T||^2, whereas reusing the forward samples adds the
+gradient of the estimator's variance.
 """
 from __future__ import annotations
 
@@ -25,8 +31,10 @@ class DiffRenderer:
     per call."""
 
     def __init__(self, scene: scenes.Scene, spp: int, max_depth: int, seed: int = 1,
-                 rr_start: int | None = None, device: int = 0, ctx: native.Context | None = None):
+                 rr_start: int | None = None, device: int = 0, ctx: native.Context | None = None,
+                 decorrelate: bool = False):
         self.scene = scene
+        self.decorrelate = decorrelate
         self.ctx = ctx or native.Context(device)
         self.ctx.upload_scene(scene)
         self.ctx.rebuild_nmh()
@@ -112,13 +120,15 @@ def _make_function():
         def forward(ctx, renderer, albedo, radiance):
             img = renderer.render(albedo, radiance)
             ctx.renderer = renderer
+            # the forward's seed, or an independent one for a decorrelated replay
+            ctx.seed = (renderer.seed ^ 0x5BD1E995) & 0xFFFFFFFF if renderer.decorrelate else renderer.seed
             ctx.save_for_backward(albedo.detach().clone(), radiance.detach().clone())
             return img[:, :3].contiguous()
 
         @staticmethod
         def backward(ctx, grad_out):
             albedo, radiance = ctx.saved_tensors
-            ga, gr = ctx.renderer.adjoint(albedo, radiance, grad_out.contiguous())
+            ga, gr = ctx.renderer.adjoint(albedo, radiance, grad_out.contiguous(), seed=ctx.seed)
             return None, ga, gr
 
     return _Render

@@ -317,8 +317,155 @@ def workload_render_c2(a) -> dict:
             "config": {"workload": "c2: 1920x1080, depth 8, RR from bounce 3", **res}}
 
 
+def c5_scene():
+    from aurora_amd import scenes
+
+    # Cornell box with 20 subdivision-4 spheres (102,424 triangles), every wall
+    # triangle and sphere half its own Lambert shader (64), 4 ceiling lights
+    return scenes.cornell(n_spheres=20, subdivisions=4, width=512, height=512, spp=64, max_depth=6,
+                          n_lights=4, split_materials=True)
+
+
+def workload_adjoint_c5(a) -> dict:
+    """Adjoint (gradient of an L2 image loss w.r.t. 64 RGB albedos and 4 RGB
+    light radiances) vs forward render at equal spp; central-difference
+    check (h = 1e-3, common random numbers) of the largest gradients; 100
+    Adam steps from a +-30% perturbation toward a 1024-spp target."""
+    import torch
+
+    from aurora_amd import diff
+
+    scene = c5_scene()
+    spp, depth = a.spp or 64, scene.max_depth
+    log(f"c5: {scene.count} triangles, {scene.shaders.shape[0]} shaders, {scene.lights.shape[0]} lights")
+    r = diff.DiffRenderer(scene, spp=spp, max_depth=depth, seed=7)
+    if a.max_paths:
+        r.ctx.set_render_params(a.max_paths)
+    gt_a = torch.tensor(scene.shaders["diffuse_color"], device="cuda")
+    gt_r = torch.tensor(scene.lights["color"], device="cuda")
+    t0 = time.perf_counter()
+    target = r.render(gt_a, gt_r, spp=1024, seed=999)[:, :3].contiguous()
+    log(f"target 1024 spp in {time.perf_counter() - t0:.2f}s")
+    g = torch.Generator(device="cpu").manual_seed(5)
+    p0a = (gt_a.cpu() * (1 + 0.3 * (2 * torch.rand(gt_a.shape, generator=g) - 1))).cuda()
+    p0r = (gt_r.cpu() * (1 + 0.3 * (2 * torch.rand(gt_r.shape, generator=g) - 1))).cuda()
+    npix = scene.width * scene.height
+
+    # forward vs adjoint time at equal spp
+    img = r.render(p0a, p0r)
+    d_img = (2.0 / (npix * 3)) * (img[:, :3] - target)
+    r.adjoint(p0a, p0r, d_img)  # warm-up
+    fw, bw = [], []
+    for _ in range(max(3, a.steps // 4)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.ctx.frame_render(scene.camera, r.opts())
+        fw.append(time.perf_counter() - t0)
+        rays_fwd = r.ctx.rays_traced()
+        t0 = time.perf_counter()
+        ga, gr = r.adjoint(p0a, p0r, d_img)
+        bw.append(time.perf_counter() - t0)
+    t_fwd, t_adj = float(np.median(fw)), float(np.median(bw))
+    log(f"forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms at {spp} spp "
+        f"({rays_fwd / t_fwd / 1e6:.0f} Mrays/s forward)")
+
+    # central differences, h = 1e-3, same seed (common random numbers)
+    def loss_at(pa, pr):
+        im = r.render(pa, pr)[:, :3]
+        return float(((im.double() - target.double()) ** 2).mean())
+
+    h = 1e-3
+    fd_rows = []
+    order_a = torch.argsort(ga.abs().flatten(), descending=True)[:8].tolist()
+    order_r = torch.argsort(gr.abs().flatten(), descending=True)[:4].tolist()
+    for kind, idx_list in (("albedo", order_a), ("radiance", order_r)):
+        for idx in idx_list:
+            m, c = divmod(idx, 3)
+            pp_a, pm_a, pp_r, pm_r = p0a.clone(), p0a.clone(), p0r.clone(), p0r.clone()
+            if kind == "albedo":
+                pp_a[m, c] += h
+                pm_a[m, c] -= h
+                grad = float(ga[m, c])
+            else:
+                pp_r[m, c] += h
+                pm_r[m, c] -= h
+                grad = float(gr[m, c])
+            fd = (loss_at(pp_a, pp_r) - loss_at(pm_a, pm_r)) / (2 * h)
+            fd_rows.append({"param": f"{kind}[{m}][{c}]", "adjoint": grad, "fd": fd,
+                            "rel_err": abs(grad - fd) / max(abs(fd), 1e-30)})
+    max_fd_err = max(row["rel_err"] for row in fd_rows)
+    log(f"central differences: max rel err {max_fd_err:.2e} over {len(fd_rows)} params")
+
+    # visible parameters: those the image depends on at all (gradient of the
+    # image sum); albedos of hidden faces cannot be recovered from the image
+    vis_a, vis_r = r.adjoint(p0a, p0r, torch.ones((npix, 3), device="cuda"))
+    vis_a = vis_a > 1e-4 * float(vis_a.max())
+    vis_r = vis_r > 1e-4 * float(vis_r.max())
+
+    def eval_loss(pa, pr):  # 256-spp loss with a fixed seed, outside the fit
+        im = r.render(pa, pr, spp=256, seed=4242)[:, :3]
+        return float(((im.double() - target.double()) ** 2).mean())
+
+    # 100 Adam steps, 16 spp per step, fresh seed per step, decorrelated
+    # backward replay (unbiased gradient of ||E[I] - T||^2)
+    steps_spp = 16
+    pa = p0a.clone().requires_grad_(True)
+    pr = p0r.clone().requires_grad_(True)
+    lr_r = 0.01 * float(p0r.mean())  # same relative step for radiances (~15) as for albedos
+    opt = torch.optim.Adam([{"params": [pa], "lr": 0.01}, {"params": [pr], "lr": lr_r}])
+    r.spp = steps_spp
+    r.decorrelate = True
+    r.ctx.frame_create(scene.width, scene.height, steps_spp)
+    losses = []
+    t0 = time.perf_counter()
+    for it in range(100):
+        r.seed = 1000 + it
+        opt.zero_grad()
+        loss = ((diff.render(r, pa, pr) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            pa.clamp_(0.01, 1.0)
+            pr.clamp_(0.0, None)
+        losses.append(float(loss.detach()))
+    t_adam = time.perf_counter() - t0
+
+    def rel_err(p, gt, vis):
+        return float(((p - gt).abs() / gt)[vis].mean())
+
+    err0_a, err1_a = rel_err(p0a, gt_a, vis_a), rel_err(pa.detach(), gt_a, vis_a)
+    err0_r, err1_r = rel_err(p0r, gt_r, vis_r), rel_err(pr.detach(), gt_r, vis_r)
+    ev0, ev1 = eval_loss(p0a, p0r), eval_loss(pa.detach(), pr.detach())
+    ev_gt = eval_loss(gt_a, gt_r)
+    log(f"Adam: 16-spp loss {losses[0]:.3e} -> {losses[-1]:.3e} in {t_adam:.1f}s; 256-spp loss "
+        f"{ev0:.3e} -> {ev1:.3e} (ground truth {ev_gt:.3e}); visible albedo rel err "
+        f"{err0_a:.3f} -> {err1_a:.3f} ({int(vis_a.sum())} channels), radiance {err0_r:.3f} -> {err1_r:.3f}")
+    r.close()
+    ratio = t_adj / t_fwd
+    return {
+        "metric": "adjoint/fwd time (c5)", "value": round(ratio, 3), "unit": "x",
+        "n_gpus": 1, "steps": len(fw), "higher_is_better": False, "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Cornell box + 20 icospheres, 64 shaders, 4 lights)",
+        "config": {
+            "workload": "c5: 512x512, depth 6 (no Russian roulette), L2 loss vs 1024-spp target",
+            "triangles": scene.count, "spp": spp, "forward_ms": round(t_fwd * 1e3, 2),
+            "adjoint_ms": round(t_adj * 1e3, 2), "forward_mrays": round(rays_fwd / t_fwd / 1e6, 1),
+            "fd_h": h, "fd_max_rel_err": max_fd_err, "fd": fd_rows,
+            "adam": {"steps": 100, "spp_per_step": steps_spp, "lr_albedo": 0.01, "lr_radiance": round(lr_r, 4), "seconds": round(t_adam, 2),
+                     "decorrelated_backward": True,
+                     "loss16_first": losses[0], "loss16_last": losses[-1],
+                     "loss16_curve_every_10": losses[::10],
+                     "loss256_start_end_groundtruth": [ev0, ev1, ev_gt],
+                     "visible_albedo_channels": int(vis_a.sum()),
+                     "albedo_rel_err_start_end": [round(err0_a, 4), round(err1_a, 4)],
+                     "radiance_rel_err_start_end": [round(err0_r, 4), round(err1_r, 4)]},
+        },
+    }
+
+
 WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m,
-             "render_c1": workload_render_c1, "render_c2": workload_render_c2}
+             "render_c1": workload_render_c1, "render_c2": workload_render_c2,
+             "adjoint_c5": workload_adjoint_c5}
 
 
 def main() -> None:
