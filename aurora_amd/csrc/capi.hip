@@ -74,7 +74,9 @@ aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* p) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (!p) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: null");
   if (p->max_paths > (1u << 26)) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: max_paths > 2^26");
+  if (p->graph > 1) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: graph must be 0 or 1");
   ctx->render_params.max_paths = p->max_paths ? p->max_paths : aur::kDefaultMaxPaths;
+  ctx->render_params.graph = p->graph;
   return AUR_OK;
 }
 

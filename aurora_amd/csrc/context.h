@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/aurora_amd.h"
 #include "common.h"
@@ -21,6 +23,10 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -47,6 +53,34 @@ struct DevBuf {
 // path state plus 48 B per next-event slot and path).
 constexpr uint32_t kDefaultMaxPaths = 1u << 21;
 
+// One render job of the PBR wavefront: a camera over one built scene.  A
+// frame render is a single job; a batch is many.  Paths are numbered
+// globally: the job owns [path_start, path_start + paths), sample-major, so
+// global path g is pixel (g - path_start) % npix of sample
+// (g - path_start) / npix.
+struct JobDev {
+  const NodeRec* nodes;
+  const float* normals;  // tree order, 9 floats per triangle
+  const uint16_t* shader_ids;
+  const aur_shader* shaders;
+  const aur_light* lights;
+  float4* pixels;  // accumulation, npix float4
+  unsigned long long path_start;
+  uint32_t n_nodes, num_shaders, num_lights, nslots;
+  uint32_t w, h, npix, spp, max_depth, rr_start, seed, paths;
+  aur_camera cam;
+};
+
+// A built scene kept for batches: tree-order arrays as the frame render uses.
+struct SceneStore {
+  uint32_t count = 0, levels = 0, num_shaders = 0, num_lights = 0, nslots = 0;
+  DevBuf<NodeRec> nodes;
+  DevBuf<float> normals;
+  DevBuf<uint16_t> shader_ids;
+  DevBuf<aur_shader> shaders;
+  DevBuf<aur_light> lights;
+};
+
 struct FrameState {
   uint32_t width = 0, height = 0, spp = 0;
   DevBuf<float4> pixels;  // RGBA accumulation (framebuffer)
@@ -63,6 +97,10 @@ struct FrameState {
   DevBuf<float> rec_rr, rec_w;
   DevBuf<double> adj_partials, adj_total;
   DevBuf<uint32_t> adj_absmax;
+  // batches: job table, job index per path of a pass, pass start
+  DevBuf<JobDev> jobs;
+  DevBuf<uint32_t> path_job;
+  DevBuf<unsigned long long> pass_g0;
   size_t shadow_cap = 0;
   uint64_t rays_traced = 0;
 };
@@ -78,7 +116,7 @@ struct aur_context {
   // limited residency: fastest shape measured at 1M triangles / 2^24 rays
   aur_trace_params trace_params{1, 256, 16, 0, 0};
   aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
-  aur_render_params render_params{aur::kDefaultMaxPaths};
+  aur_render_params render_params{aur::kDefaultMaxPaths, 0};
 
   // uploaded geometry (input order)
   uint32_t count = 0;
@@ -103,6 +141,9 @@ struct aur_context {
   uint32_t num_slots = 0;  // next-event slots per path vertex
 
   aur::FrameState frame;
+
+  // built scenes kept for batches (aur_scene_add)
+  std::vector<std::unique_ptr<aur::SceneStore>> scenes;
 
   aur_status fail(aur_status code, const std::string& msg) {
     last_error = msg;
@@ -138,4 +179,18 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
                         uint32_t s0, uint32_t s1);
 aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
                          const float* d_image, float* g_albedo, float* g_light);
+
+// PBR wavefront passes over the global path range [begin, end) of `jobs`
+// (host array; one job renders with wave-uniform parameters, several share
+// every pass).  cap: paths per pass; levels, nslots, max_depth: maxima over
+// the jobs.  pass_end_ms (optional, host, one entry per pass): device time
+// from the start to the end of each pass.
+struct PbrRun {
+  const JobDev* jobs = nullptr;
+  uint32_t njobs = 0;
+  unsigned long long begin = 0, end = 0;
+  uint32_t cap = 0, levels = 0, nslots = 0, max_depth = 0;
+  std::vector<float>* pass_end_ms = nullptr;
+};
+aur_status pbr_run(aur_context* ctx, const PbrRun& run, unsigned long long* stats);
 }  // namespace aur

@@ -303,11 +303,19 @@ __device__ inline void onb(V3 n, V3& t1, V3& t2) {
   t2 = mk(b, sign + n.y * n.y * a, -n.y);
 }
 
-// One pass traces `batch` consecutive samples of every pixel: path q is pixel
-// q % npix, sample sample + q / npix.  Per-path arrays are indexed by q.
+// Wavefront state shared by the PBR kernels.  A pass traces the global paths
+// [g0, g0 + n), n = min(cap, end - g0), with g0 read from device memory so a
+// captured pass replays unchanged; per-path arrays are indexed by q = g - g0.
+// kMulti = false: one job, passed by value (wave-uniform parameters);
+// kMulti = true: jobs[path_job[q]].
 struct PbrArgs {
-  uint32_t w, h, spp, max_depth, rr_start, seed, sample, nslots;
-  uint32_t npix, batch;
+  JobDev job;
+  const JobDev* jobs;
+  uint32_t njobs;
+  uint32_t* path_job;
+  const unsigned long long* g0p;
+  unsigned long long end;
+  uint32_t cap, nslots, max_depth;
   float4 *ray_o, *ray_d, *beta, *radiance;
   aur_hit* hits;
   float4 *sh_o, *sh_d, *sh_c;
@@ -315,44 +323,83 @@ struct PbrArgs {
   uint32_t* counters;  // [0],[1] queue lengths
   unsigned long long* stats;
   // path-vertex records of an adjoint replay (null in a plain render),
-  // indexed [bounce][path] and [bounce][slot][path]: shader index of the hit
-  // (-1 ends the path), Russian-roulette survival probability (1 when not
-  // applied) and each next-event slot's weight w, zero when occluded, with
+  // indexed [bounce][q] and [bounce][slot][q] with stride cap: shader index of
+  // the hit (-1 ends the path), Russian-roulette survival probability (1 when
+  // not applied) and each next-event slot's weight w, zero when occluded, with
   // contribution = beta * albedo * light radiance * w.
   int32_t* rec_mat;
   float* rec_rr;
   float* rec_w;
 };
 
-__device__ __forceinline__ uint32_t path_pixel(const PbrArgs& a, uint32_t q) { return q % a.npix; }
-__device__ __forceinline__ uint32_t path_sample(const PbrArgs& a, uint32_t q) {
-  return a.sample + q / a.npix;
+__device__ __forceinline__ uint32_t pass_paths(const PbrArgs& a, unsigned long long g0) {
+  const unsigned long long left = a.end > g0 ? a.end - g0 : 0ull;
+  return left < a.cap ? (uint32_t)left : a.cap;
 }
 
-__global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ queue) {
+template <bool kMulti>
+__device__ __forceinline__ const JobDev& job_at(const PbrArgs& a, uint32_t jb) {
+  if constexpr (kMulti)
+    return a.jobs[jb];
+  else
+    return a.job;
+}
+
+template <bool kMulti>
+__device__ __forceinline__ const JobDev& job_of(const PbrArgs& a, uint32_t q) {
+  if constexpr (kMulti)
+    return a.jobs[a.path_job[q]];
+  else
+    return a.job;
+}
+
+__device__ __forceinline__ uint32_t job_shader(const JobDev& J, uint32_t tri) {
+  uint32_t sid = J.shader_ids[tri];
+  sid = sid > 0 ? sid - 1 : 0;  // getSafeID (src/util/array.h:18)
+  return sid < J.num_shaders ? sid : 0;
+}
+
+template <bool kMulti>
+__global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t npaths = a.npix * a.batch;
+  const unsigned long long g0 = *a.g0p;
+  const uint32_t n = pass_paths(a, g0);
   if (q == 0) {
-    a.counters[0] = npaths;
+    a.counters[0] = n;
     a.counters[1] = 0;
   }
-  if (q >= npaths) return;
-  const uint32_t p = path_pixel(a, q), smp = path_sample(a, q);
-  const uint32_t x = p % a.w, y = p / a.w;
-  const U4 r = pbr_draw(a.seed, p, smp, 0, 0);
-  const float j0 = u01(r.x), j1 = u01(r.y);
+  if (q >= n) return;
+  const unsigned long long g = g0 + q;
+  uint32_t jb = 0;
+  if constexpr (kMulti) {
+    // last job whose range starts at or before g
+    uint32_t lo = 0, hi = a.njobs;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.jobs[mid].path_start <= g) lo = mid; else hi = mid;
+    }
+    jb = lo;
+    a.path_job[q] = jb;
+  }
+  const JobDev& J = job_at<kMulti>(a, jb);
+  const uint32_t r = (uint32_t)(g - J.path_start);
+  const uint32_t p = r % J.npix, smp = r / J.npix;
+  const uint32_t x = p % J.w, y = p / J.w;
+  const U4 rn = pbr_draw(J.seed, p, smp, 0, 0);
+  const float j0 = u01(rn.x), j1 = u01(rn.y);
   uint32_t sw = 1;
-  while ((sw + 1) * (sw + 1) <= a.spp) ++sw;
+  while ((sw + 1) * (sw + 1) <= J.spp) ++sw;
   float fx, fy;
-  if (sw * sw == a.spp) {
+  if (sw * sw == J.spp) {
     fx = ((float)(smp % sw) + j0) / (float)sw;
     fy = ((float)(smp / sw) + j1) / (float)sw;
   } else {
     fx = j0;
     fy = j1;
   }
-  const float px = ((float)x + fx) / (float)a.w - 0.5f;
-  const float py = ((float)y + fy) / (float)a.h - 0.5f;
+  const aur_camera& cam = J.cam;
+  const float px = ((float)x + fx) / (float)J.w - 0.5f;
+  const float py = ((float)y + fy) / (float)J.h - 0.5f;
   const V3 d = normalize(px * ld3(cam.right) * cam.tanfov[0] + py * ld3(cam.up) * cam.tanfov[1] +
                          ld3(cam.forward));
   a.ray_o[q] = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
@@ -363,7 +410,8 @@ __global__ void k_pbr_camera(PbrArgs a, aur_camera cam, uint32_t* __restrict__ q
 }
 
 // Closest hit for every queued path: persistent waves over the queue.
-__global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, int cin,
+template <bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
                                                        const uint32_t* __restrict__ queue) {
   extern __shared__ float2 stk_mem[];
   const uint32_t n_in = a.counters[cin];
@@ -374,9 +422,13 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, i
     atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
   persistent_trace<false, kRefill>(
-      sc.nodes, sc.n_nodes, n_in, lds_stack(stk_mem + threadIdx.x, blockDim.x),
-      [&](uint64_t j, V3& o, V3& d, float& tmax) {
+      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+      [&](uint64_t j, V3& o, V3& d, float& tmax, TreeRef& tree) {
         const uint32_t p = queue[j];
+        if constexpr (kMulti) {
+          const JobDev& J = a.jobs[a.path_job[p]];
+          tree = TreeRef{J.nodes, J.n_nodes};
+        }
         const float4 o4 = a.ray_o[p], d4 = a.ray_d[p];
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
@@ -386,31 +438,34 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, SceneDev sc, i
       [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
 }
 
-template <bool kRecord>
-__global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
-                            const uint32_t* __restrict__ qin, uint32_t* __restrict__ qout) {
+template <bool kMulti, bool kRecord>
+__global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t* __restrict__ qin,
+                            uint32_t* __restrict__ qout) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_in = a.counters[cin];
   if (j >= n_in) return;
-  const uint32_t p = qin[j];  // path index
-  const uint32_t pix = path_pixel(a, p), smp = path_sample(a, p);
+  const uint32_t p = qin[j];  // path index in the pass
+  const JobDev& J = job_of<kMulti>(a, p);
+  const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
+  const uint32_t pix = r0 % J.npix, smp = r0 / J.npix;
   const size_t base = (size_t)j * a.nslots;
-  const size_t np = (size_t)a.npix * a.batch;
+  const size_t np = a.cap;
   const aur_hit hit = a.hits[p];
   if (hit.id < 0) {
     for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s].w = 0.0f;
     if (kRecord) a.rec_mat[bounce * np + p] = -1;
     return;
   }
-  if (kRecord) a.rec_mat[bounce * np + p] = (int32_t)shader_index(sc, (uint32_t)hit.id);
+  const uint32_t sid = job_shader(J, (uint32_t)hit.id);
+  if (kRecord) a.rec_mat[bounce * np + p] = (int32_t)sid;
   const float4 o4 = a.ray_o[p], d4 = a.ray_d[p], b4 = a.beta[p];
   float4 L4 = a.radiance[p];
   const V3 O = mk(o4.x, o4.y, o4.z), D = mk(d4.x, d4.y, d4.z);
   V3 beta = mk(b4.x, b4.y, b4.z);
   const V3 P = O + (D * hit.t);
   const V3 wo = -D;
-  const aur_shader& sh = shader_of(sc, (uint32_t)hit.id);
-  V3 N = normalize(interp9(sc.normals, (uint32_t)hit.id, hit.u, hit.v));
+  const aur_shader& sh = J.shaders[sid];
+  V3 N = normalize(interp9(J.normals, (uint32_t)hit.id, hit.u, hit.v));
   if (dot(N, wo) < 0.0f) N = -N;
   const V3 albedo = ld3(sh.diffuse_color) * sh.diffuse;
   const V3 f = albedo * kInvPi;
@@ -418,14 +473,14 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
   a.radiance[p] = make_float4(L.x, L.y, L.z, bounce == 0 ? 1.0f : L4.w);
   const V3 bf = beta * f;
   uint32_t slot = 16, e = 0;
-  for (uint32_t li = 0; li < sc.num_lights; ++li) {
-    const aur_light& Lt = sc.lights[li];
+  for (uint32_t li = 0; li < J.num_lights; ++li) {
+    const aur_light& Lt = J.lights[li];
     const V3 CI = ld3(Lt.color) * Lt.intensity;
     if (Lt.type == AUR_LIGHT_AREA) {
       const int ns = Lt.samples > 0 ? Lt.samples : 1;
       const float inv = 1.0f / (float)ns;
       for (int k = 0; k < ns; ++k, ++e) {
-        const U4 r = pbr_draw(a.seed, pix, smp, bounce, slot++);
+        const U4 r = pbr_draw(J.seed, pix, smp, bounce, slot++);
         const V3 Q = ld3(Lt.position) + ((ld3(Lt.e1) * u01(r.x)) + (ld3(Lt.e2) * u01(r.y)));
         V3 wi = Q - P;
         const float dist = length(wi);
@@ -476,10 +531,13 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
       ++e;
     }
   }
-  if (bounce + 1 >= a.max_depth) return;
-  const U4 r = pbr_draw(a.seed, pix, smp, bounce, 1);
+  if constexpr (kMulti) {
+    for (; e < a.nslots; ++e) a.sh_d[base + e].w = 0.0f;  // slots of jobs with more lights
+  }
+  if (bounce + 1 >= J.max_depth) return;
+  const U4 r = pbr_draw(J.seed, pix, smp, bounce, 1);
   if (kRecord) a.rec_rr[bounce * np + p] = 1.0f;
-  if (bounce >= a.rr_start) {
+  if (bounce >= J.rr_start) {
     const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
     if (u01(r.z) >= q) {
       if (kRecord) a.rec_mat[(bounce + 1) * np + p] = -1;
@@ -510,15 +568,21 @@ __global__ void k_pbr_shade(PbrArgs a, SceneDev sc, int cin, uint32_t bounce,
 }
 
 // Any hit for every next-event slot that needs a shadow ray.
-__global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, SceneDev sc, int cin) {
+template <bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
+                                                       const uint32_t* __restrict__ qin) {
   extern __shared__ float2 stk_mem[];
   const uint64_t total = (uint64_t)a.counters[cin] * a.nslots;
   uint32_t traced = 0;
   persistent_trace<true, kRefill>(
-      sc.nodes, sc.n_nodes, total, lds_stack(stk_mem + threadIdx.x, blockDim.x),
-      [&](uint64_t e, V3& o, V3& d, float& tmax) {
+      TreeRef{a.job.nodes, a.job.n_nodes}, total, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+      [&](uint64_t e, V3& o, V3& d, float& tmax, TreeRef& tree) {
         const float4 d4 = a.sh_d[e];
         if (d4.w != 1.0f) return false;
+        if constexpr (kMulti) {
+          const JobDev& J = a.jobs[a.path_job[qin[e / a.nslots]]];
+          tree = TreeRef{J.nodes, J.n_nodes};
+        }
         const float4 o4 = a.sh_o[e];
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
@@ -537,7 +601,7 @@ __global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_
   const uint32_t p = qin[j];
   if (a.hits[p].id < 0) return;
   const size_t base = (size_t)j * a.nslots;
-  const size_t np = (size_t)a.npix * a.batch;
+  const size_t np = a.cap;
   float4 L4 = a.radiance[p];
   V3 L = mk(L4.x, L4.y, L4.z);
   for (uint32_t s = 0; s < a.nslots; ++s) {
@@ -554,14 +618,22 @@ __global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_
 }
 
 // Adds the pass's samples to each pixel in sample order, so the sum does not
-// depend on how samples were grouped into passes.
-__global__ void k_pbr_accumulate(PbrArgs a, float4* __restrict__ pixels) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.npix) return;
-  const float weight = 1.0f / (float)a.spp;
-  float4 px = pixels[p];
-  for (uint32_t b = 0; b < a.batch; ++b) {
-    const float4 L = a.radiance[p + (size_t)b * a.npix];
+// depend on how paths were grouped into passes.  The thread of a pixel's
+// first path in the pass walks that pixel's later samples (stride npix).
+template <bool kMulti>
+__global__ void k_pbr_accumulate(PbrArgs a) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long g0 = *a.g0p;
+  const uint32_t n = pass_paths(a, g0);
+  if (q >= n) return;
+  const JobDev& J = job_of<kMulti>(a, q);
+  const uint32_t r = (uint32_t)(g0 + q - J.path_start);
+  if (q >= J.npix && r >= J.npix) return;  // the pixel's previous sample is in this pass
+  const uint32_t pix = r % J.npix;
+  const float weight = 1.0f / (float)J.spp;
+  float4 px = J.pixels[pix];
+  for (uint32_t qq = q, rr = r; qq < n && rr < J.paths; qq += J.npix, rr += J.npix) {
+    const float4 L = a.radiance[qq];
     if (L.w != 0.0f) {
       px.x += L.x * weight;
       px.y += L.y * weight;
@@ -569,7 +641,25 @@ __global__ void k_pbr_accumulate(PbrArgs a, float4* __restrict__ pixels) {
       px.w += weight;
     }
   }
-  pixels[p] = px;
+  J.pixels[pix] = px;
+}
+
+// Next pass: g0 += cap (a kernel, so that a captured pass replays).
+__global__ void k_pbr_advance(unsigned long long* g0p, uint32_t cap) {
+  if (threadIdx.x == 0) *g0p += cap;
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+// Clears each job's accumulation buffer (block b: jobs b, b + grid, ...).
+__global__ void k_clear_jobs(const JobDev* __restrict__ jobs, uint32_t njobs) {
+  for (uint32_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+    float4* px = jobs[jb].pixels;
+    const uint32_t n = jobs[jb].npix;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) px[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 // ---------------------------------------------------------------- adjoint --
@@ -609,21 +699,26 @@ __device__ inline void adj_add(unsigned long long* acc, uint32_t i, float v, flo
   atomicAdd(acc + i, (unsigned long long)__float2ll_rn(x));
 }
 
+// One job (the frame); block partials[block][param] as doubles.
 template <int kMax>
-__global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, SceneDev sc, const float4* __restrict__ dimg,
+__global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, const float4* __restrict__ dimg,
                                                    const uint32_t* __restrict__ absmax,
                                                    double* __restrict__ partials, uint32_t nparams) {
   extern __shared__ unsigned long long acc[];
   for (uint32_t i = threadIdx.x; i < nparams; i += blockDim.x) acc[i] = 0ull;
   __syncthreads();
+  const JobDev& J = a.job;
   const float dmax = __uint_as_float(*absmax);
-  const float scale = dmax > 0.0f ? kAdjQuant * (float)a.spp / dmax : 0.0f;
-  const float inv_spp = 1.0f / (float)a.spp;
-  const uint32_t np = a.npix * a.batch;
-  const uint32_t lbase = 3u * sc.num_shaders;
+  const float scale = dmax > 0.0f ? kAdjQuant * (float)J.spp / dmax : 0.0f;
+  const float inv_spp = 1.0f / (float)J.spp;
+  const unsigned long long g0 = *a.g0p;
+  const uint32_t np = pass_paths(a, g0);
+  const size_t stride = a.cap;
+  const uint32_t lbase = 3u * J.num_shaders;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; scale > 0.0f && q < np;
        q += gridDim.x * blockDim.x) {
-    const float4 g = dimg[path_pixel(a, q)];
+    const uint32_t pix = (uint32_t)(g0 + q - J.path_start) % J.npix;
+    const float4 g = dimg[pix];
     const V3 dl = mk(g.x * inv_spp, g.y * inv_spp, g.z * inv_spp);
     if (dl.x == 0.0f && dl.y == 0.0f && dl.z == 0.0f) continue;
     V3 beta = mk(1.0f, 1.0f, 1.0f);
@@ -633,18 +728,18 @@ __global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, SceneDev sc, const
     uint32_t n = 0;
 #pragma unroll
     for (int b = 0; b < kMax; ++b) {
-      if ((uint32_t)b < a.max_depth && n == (uint32_t)b) {
-        const int32_t m = a.rec_mat[(size_t)b * np + q];
+      if ((uint32_t)b < J.max_depth && n == (uint32_t)b) {
+        const int32_t m = a.rec_mat[(size_t)b * stride + q];
         if (m >= 0) {
-          const aur_shader& sh = sc.shaders[m];
+          const aur_shader& sh = J.shaders[m];
           const V3 rho = ld3(sh.diffuse_color) * sh.diffuse;
           V3 d = mk(0.0f, 0.0f, 0.0f);
           uint32_t s = 0;
-          for (uint32_t li = 0; li < sc.num_lights; ++li) {
-            const aur_light& Lt = sc.lights[li];
+          for (uint32_t li = 0; li < J.num_lights; ++li) {
+            const aur_light& Lt = J.lights[li];
             const uint32_t ns = Lt.type == AUR_LIGHT_AREA ? (uint32_t)max(1, (int)Lt.samples) : 1u;
             float W = 0.0f;
-            for (uint32_t k = 0; k < ns; ++k, ++s) W += a.rec_w[((size_t)b * a.nslots + s) * np + q];
+            for (uint32_t k = 0; k < ns; ++k, ++s) W += a.rec_w[((size_t)b * a.nslots + s) * stride + q];
             if (W != 0.0f) {
               const V3 CI = ld3(Lt.color) * Lt.intensity;
               d = d + CI * W;
@@ -657,8 +752,8 @@ __global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, SceneDev sc, const
           betas[b] = beta;
           D[b] = d;
           mats[b] = m;
-          const bool more = (uint32_t)b + 1 < a.max_depth;
-          rr[b] = more ? a.rec_rr[(size_t)b * np + q] : 1.0f;
+          const bool more = (uint32_t)b + 1 < J.max_depth;
+          rr[b] = more ? a.rec_rr[(size_t)b * stride + q] : 1.0f;
           beta = (beta / rr[b]) * rho;
           n = b + 1;
         }
@@ -668,7 +763,7 @@ __global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, SceneDev sc, const
 #pragma unroll
     for (int b = kMax - 1; b >= 0; --b) {
       if ((uint32_t)b < n) {
-        const aur_shader& sh = sc.shaders[mats[b]];
+        const aur_shader& sh = J.shaders[mats[b]];
         const V3 rho = ld3(sh.diffuse_color) * sh.diffuse;
         const V3 R = D[b] + G / rr[b];
         const V3 ga = dl * betas[b] * R;
@@ -711,6 +806,10 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+__global__ void k_adj_zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
 inline uint32_t nblocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 // Resident grid of a persistent kernel: occupancy-limited blocks on every CU,
@@ -725,10 +824,6 @@ uint32_t resident_grid(aur_context* ctx, const void* fn, size_t lds, uint64_t ma
   return (uint32_t)(blocks ? blocks : 1);
 }
 
-__global__ void k_adj_zero_u32(uint32_t* p) {
-  if (threadIdx.x == 0) *p = 0u;
-}
-
 constexpr uint32_t kAdjMaxDepth = 16;
 constexpr uint32_t kAdjMaxParams = 6144;  // 48 KiB of LDS accumulators
 
@@ -739,97 +834,211 @@ struct AdjointState {
   uint32_t nparams = 0;
 };
 
-// Sample passes [s0, s1) of the PBR path loop.  Without `adj` each pass is
-// accumulated into the frame; with it the passes are replayed with vertex
-// records and each is followed by the adjoint sweep and the ordered reduction
-// of the per-block partial gradients.
-aur_status pbr_passes(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o, uint32_t s0,
-                      uint32_t s1, unsigned long long* stats, AdjointState* adj) {
+// Buffers of a run: per-path state for `cap` paths and `nslots` next-event
+// slots, vertex records for the adjoint.
+aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t max_depth, bool records) {
   FrameState& fr = ctx->frame;
-  hipStream_t s = ctx->stream;
-  const uint32_t w = fr.width, h = fr.height, npix = w * h;
-  SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
-              ctx->num_shaders, ctx->lights.p, ctx->num_lights};
-  const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
-  const uint32_t nslots = ctx->num_slots;
-  // samples per pass: enough paths in flight to fill the device
-  const uint32_t max_paths = std::max<uint32_t>(ctx->render_params.max_paths, 1u);
-  const uint32_t batch_cap = std::max<uint32_t>(1u, max_paths / npix);
-  const uint32_t batch = std::min<uint32_t>(batch_cap, std::max<uint32_t>(1u, s1 - s0));
-  const uint64_t npaths = (uint64_t)npix * batch;
-  const uint64_t rec_words = adj ? npaths * o->max_depth * std::max<uint32_t>(nslots, 1u) : 0;
-  if (npaths * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull || rec_words > (1ull << 34))
-    return ctx->fail(AUR_ERR_INVALID_ARG, "render: frame too large");
-  const size_t nsh = std::max<size_t>(1, (size_t)npaths * nslots);
-  AUR_HIP(ctx, fr.ray_o.reserve(npaths));
-  AUR_HIP(ctx, fr.ray_d.reserve(npaths));
-  AUR_HIP(ctx, fr.beta.reserve(npaths));
-  AUR_HIP(ctx, fr.radiance.reserve(npaths));
-  AUR_HIP(ctx, fr.hits.reserve(npaths));
-  AUR_HIP(ctx, fr.queue[0].reserve(npaths));
-  AUR_HIP(ctx, fr.queue[1].reserve(npaths));
+  if ((uint64_t)cap * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "render: too many paths per pass");
+  const size_t nsh = std::max<size_t>(1, (size_t)cap * nslots);
+  AUR_HIP(ctx, fr.counters.reserve(8));
+  AUR_HIP(ctx, fr.pass_g0.reserve(1));
+  AUR_HIP(ctx, fr.ray_o.reserve(cap));
+  AUR_HIP(ctx, fr.ray_d.reserve(cap));
+  AUR_HIP(ctx, fr.beta.reserve(cap));
+  AUR_HIP(ctx, fr.radiance.reserve(cap));
+  AUR_HIP(ctx, fr.hits.reserve(cap));
+  AUR_HIP(ctx, fr.queue[0].reserve(cap));
+  AUR_HIP(ctx, fr.queue[1].reserve(cap));
   AUR_HIP(ctx, fr.sh_o.reserve(nsh));
   AUR_HIP(ctx, fr.sh_d.reserve(nsh));
   AUR_HIP(ctx, fr.sh_c.reserve(nsh));
   AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
-  PbrArgs a{w, h, o->spp, o->max_depth, o->rr_start, o->seed, 0, nslots, npix, batch,
-            fr.ray_o.p, fr.ray_d.p, fr.beta.p, fr.radiance.p, fr.hits.p,
-            fr.sh_o.p, fr.sh_d.p, fr.sh_c.p, fr.sh_occ.p, fr.counters.p, stats,
-            nullptr, nullptr, nullptr};
-  uint32_t sweep_grid = 0;
-  if (adj) {
-    AUR_HIP(ctx, fr.rec_mat.reserve(npaths * o->max_depth));
-    AUR_HIP(ctx, fr.rec_rr.reserve(npaths * o->max_depth));
-    AUR_HIP(ctx, fr.rec_w.reserve(std::max<uint64_t>(1, npaths * o->max_depth * nslots)));
-    a.rec_mat = fr.rec_mat.p;
-    a.rec_rr = fr.rec_rr.p;
-    a.rec_w = fr.rec_w.p;
-    // a few resident blocks per CU; each keeps LDS accumulators for all parameters
-    sweep_grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4, (npaths + 255) / 256);
-    AUR_HIP(ctx, fr.adj_partials.reserve((size_t)sweep_grid * adj->nparams));
+  if (records) {
+    AUR_HIP(ctx, fr.rec_mat.reserve((size_t)cap * max_depth));
+    AUR_HIP(ctx, fr.rec_rr.reserve((size_t)cap * max_depth));
+    AUR_HIP(ctx, fr.rec_w.reserve(std::max<size_t>(1, (size_t)cap * max_depth * nslots)));
   }
-  const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend, lds, npaths);
-  const uint32_t grid_shadow = resident_grid(ctx, (const void*)k_pbr_shadow, lds, npaths * nslots);
-  for (uint32_t smp = s0; smp < s1; smp += a.batch) {
-    a.sample = smp;
-    a.batch = std::min<uint32_t>(batch, s1 - smp);
-    const uint64_t np = (uint64_t)npix * a.batch;
-    k_pbr_camera<<<nblocks(np, 256), 256, 0, s>>>(a, *cam, fr.queue[0].p);
-    int cin = 0;
-    for (uint32_t b = 0; b < o->max_depth; ++b) {
-      uint32_t* qin = fr.queue[cin].p;
-      uint32_t* qout = fr.queue[cin ^ 1].p;
-      k_pbr_extend<<<grid_extend, kBlock, lds, s>>>(a, sc, cin, qin);
-      if (adj)
-        k_pbr_shade<true><<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
-      else
-        k_pbr_shade<false><<<nblocks(np, 128), 128, 0, s>>>(a, sc, cin, b, qin, qout);
-      if (nslots) {
-        k_pbr_shadow<<<grid_shadow, kBlock, lds, s>>>(a, sc, cin);
-        if (adj)
-          k_pbr_resolve<true><<<nblocks(np, 256), 256, 0, s>>>(a, cin, b, qin);
-        else
-          k_pbr_resolve<false><<<nblocks(np, 256), 256, 0, s>>>(a, cin, b, qin);
-      }
-      cin ^= 1;
-    }
-    if (!adj) {
-      k_pbr_accumulate<<<nblocks(npix, 256), 256, 0, s>>>(a, fr.pixels.p);
-    } else {
-      const size_t slds = (size_t)adj->nparams * sizeof(unsigned long long);
-      const uint32_t g = (uint32_t)std::min<uint64_t>(sweep_grid, (np + 255) / 256);
-      if (o->max_depth <= 8)
-        k_adj_sweep<8><<<g, 256, slds, s>>>(a, sc, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
-      else
-        k_adj_sweep<16><<<g, 256, slds, s>>>(a, sc, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
-      k_adj_reduce<<<nblocks(adj->nparams, 64), 64, 0, s>>>(fr.adj_partials.p, g, adj->nparams, adj->total);
-    }
-  }
-  AUR_LAUNCH_CHECK(ctx, "pbr kernels");
   return AUR_OK;
 }
 
+// Kernels of one pass (camera, bounce loop, accumulation or adjoint sweep,
+// advance) on the context stream.
+template <bool kMulti>
+void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_extend, uint32_t grid_shadow,
+              AdjointState* adj, uint32_t sweep_grid) {
+  FrameState& fr = ctx->frame;
+  hipStream_t s = ctx->stream;
+  const uint32_t cap = a.cap;
+  k_pbr_camera<kMulti><<<nblocks(cap, 256), 256, 0, s>>>(a, fr.queue[0].p);
+  int cin = 0;
+  for (uint32_t b = 0; b < a.max_depth; ++b) {
+    uint32_t* qin = fr.queue[cin].p;
+    uint32_t* qout = fr.queue[cin ^ 1].p;
+    k_pbr_extend<kMulti><<<grid_extend, kBlock, lds, s>>>(a, cin, qin);
+    if (adj)
+      k_pbr_shade<kMulti, true><<<nblocks(cap, 128), 128, 0, s>>>(a, cin, b, qin, qout);
+    else
+      k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, cin, b, qin, qout);
+    if (a.nslots) {
+      k_pbr_shadow<kMulti><<<grid_shadow, kBlock, lds, s>>>(a, cin, qin);
+      if (adj)
+        k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, cin, b, qin);
+      else
+        k_pbr_resolve<false><<<nblocks(cap, 256), 256, 0, s>>>(a, cin, b, qin);
+    }
+    cin ^= 1;
+  }
+  if (!adj) {
+    k_pbr_accumulate<kMulti><<<nblocks(cap, 256), 256, 0, s>>>(a);
+  } else if constexpr (!kMulti) {
+    const size_t slds = (size_t)adj->nparams * sizeof(unsigned long long);
+    if (a.max_depth <= 8)
+      k_adj_sweep<8><<<sweep_grid, 256, slds, s>>>(a, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
+    else
+      k_adj_sweep<16><<<sweep_grid, 256, slds, s>>>(a, adj->dimg, adj->absmax, fr.adj_partials.p, adj->nparams);
+    k_adj_reduce<<<nblocks(adj->nparams, 64), 64, 0, s>>>(fr.adj_partials.p, sweep_grid, adj->nparams, adj->total);
+  }
+  k_pbr_advance<<<1, 64, 0, s>>>(fr.pass_g0.p, cap);
+}
+
+// Every pass of a run, launched directly or as replays of one captured pass.
+template <bool kMulti>
+aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* stats, AdjointState* adj) {
+  FrameState& fr = ctx->frame;
+  hipStream_t s = ctx->stream;
+  const uint32_t cap = run.cap;
+  aur_status st = pbr_reserve(ctx, cap, run.nslots, run.max_depth, adj != nullptr);
+  if (st != AUR_OK) return st;
+  PbrArgs a{};
+  a.job = run.jobs[0];
+  a.njobs = run.njobs;
+  if (kMulti) {
+    AUR_HIP(ctx, fr.jobs.reserve(run.njobs));
+    AUR_HIP(ctx, fr.path_job.reserve(cap));
+    AUR_HIP(ctx, hipMemcpyAsync(fr.jobs.p, run.jobs, sizeof(JobDev) * run.njobs, hipMemcpyHostToDevice, s));
+    AUR_HIP(ctx, hipStreamSynchronize(s));  // run.jobs is pageable host memory
+    a.jobs = fr.jobs.p;
+    a.path_job = fr.path_job.p;
+    // a batch renders its jobs whole: clear their accumulation buffers
+    k_clear_jobs<<<std::min<uint32_t>(run.njobs, 1024), 256, 0, s>>>(fr.jobs.p, run.njobs);
+  }
+  a.g0p = fr.pass_g0.p;
+  a.end = run.end;
+  a.cap = cap;
+  a.nslots = run.nslots;
+  a.max_depth = run.max_depth;
+  a.ray_o = fr.ray_o.p;
+  a.ray_d = fr.ray_d.p;
+  a.beta = fr.beta.p;
+  a.radiance = fr.radiance.p;
+  a.hits = fr.hits.p;
+  a.sh_o = fr.sh_o.p;
+  a.sh_d = fr.sh_d.p;
+  a.sh_c = fr.sh_c.p;
+  a.sh_occ = fr.sh_occ.p;
+  a.counters = fr.counters.p;
+  a.stats = stats;
+  if (adj) {
+    a.rec_mat = fr.rec_mat.p;
+    a.rec_rr = fr.rec_rr.p;
+    a.rec_w = fr.rec_w.p;
+  }
+  const size_t lds = (size_t)run.levels * kBlock * sizeof(float2);
+  const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend<kMulti>, lds, cap);
+  const uint32_t grid_shadow =
+      resident_grid(ctx, (const void*)k_pbr_shadow<kMulti>, lds, (uint64_t)cap * run.nslots);
+  uint32_t sweep_grid = 0;
+  if (adj) {
+    // a few resident blocks per CU; each keeps LDS accumulators for all parameters
+    sweep_grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4, (cap + 255) / 256);
+    AUR_HIP(ctx, fr.adj_partials.reserve((size_t)sweep_grid * adj->nparams));
+  }
+  const uint64_t npass = (run.end - run.begin + cap - 1) / cap;
+  k_set_u64<<<1, 64, 0, s>>>(fr.pass_g0.p, run.begin);
+  AUR_LAUNCH_CHECK(ctx, "pbr setup");
+  std::vector<hipEvent_t> ev;
+  if (run.pass_end_ms) {
+    ev.resize(npass + 1);
+    for (auto& e : ev) AUR_HIP(ctx, hipEventCreate(&e));
+    AUR_HIP(ctx, hipEventRecord(ev[0], s));
+  }
+  aur_status result = AUR_OK;
+  const bool use_graph = ctx->render_params.graph && npass > 1;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      pbr_pass<kMulti>(ctx, a, lds, grid_extend, grid_shadow, adj, sweep_grid);
+      e = hipStreamEndCapture(s, &graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) result = ctx->hip_fail(e, "pbr graph capture");
+  }
+  for (uint64_t i = 0; result == AUR_OK && i < npass; ++i) {
+    if (exec) {
+      const hipError_t e = hipGraphLaunch(exec, s);
+      if (e != hipSuccess) result = ctx->hip_fail(e, "hipGraphLaunch");
+    } else {
+      pbr_pass<kMulti>(ctx, a, lds, grid_extend, grid_shadow, adj, sweep_grid);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) result = ctx->hip_fail(e, "pbr kernels");
+    }
+    if (result == AUR_OK && run.pass_end_ms) {
+      const hipError_t e = hipEventRecord(ev[i + 1], s);
+      if (e != hipSuccess) result = ctx->hip_fail(e, "hipEventRecord");
+    }
+  }
+  if (exec) {
+    (void)hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(exec);
+  }
+  if (run.pass_end_ms) {
+    (void)hipStreamSynchronize(s);
+    run.pass_end_ms->assign(npass, 0.0f);
+    for (uint64_t i = 0; result == AUR_OK && i < npass; ++i)
+      (void)hipEventElapsedTime(&(*run.pass_end_ms)[i], ev[0], ev[i + 1]);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  return result;
+}
+
+// The frame as a single job over the context's current scene.
+JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o) {
+  FrameState& fr = ctx->frame;
+  JobDev J{};
+  J.nodes = ctx->nodes.p;
+  J.normals = ctx->normals.p;
+  J.shader_ids = ctx->shader_ids.p;
+  J.shaders = ctx->shaders.p;
+  J.lights = ctx->lights.p;
+  J.pixels = fr.pixels.p;
+  J.path_start = 0;
+  J.n_nodes = ctx->count / 2;
+  J.num_shaders = ctx->num_shaders;
+  J.num_lights = ctx->num_lights;
+  J.nslots = ctx->num_slots;
+  J.w = fr.width;
+  J.h = fr.height;
+  J.npix = fr.width * fr.height;
+  J.spp = o->spp;
+  J.max_depth = o->max_depth;
+  J.rr_start = o->rr_start;
+  J.seed = o->seed;
+  J.paths = J.npix * o->spp;
+  J.cam = *cam;
+  return J;
+}
+
 }  // namespace
+
+aur_status pbr_run(aur_context* ctx, const PbrRun& run, unsigned long long* stats) {
+  if (run.njobs == 0 || run.end <= run.begin) return AUR_OK;
+  return run.njobs == 1 ? pbr_passes<false>(ctx, run, stats, nullptr)
+                        : pbr_passes<true>(ctx, run, stats, nullptr);
+}
 
 aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o,
                         uint32_t s0, uint32_t s1) {
@@ -842,20 +1051,33 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   if (o->mode != AUR_MODE_COMPAT && o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "render: mode");
   hipStream_t s = ctx->stream;
   const uint32_t w = fr.width, h = fr.height, npix = w * h;
-  SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
-              ctx->num_shaders, ctx->lights.p, ctx->num_lights};
-  const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
+  if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
   AUR_HIP(ctx, fr.counters.reserve(8));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   if (o->mode == AUR_MODE_COMPAT) {
+    SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
+                ctx->num_shaders, ctx->lights.p, ctx->num_lights};
+    const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
     for (uint32_t smp = s0; smp < s1; ++smp) {
       k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, o->spp, smp, o->seed,
                                                           fr.pixels.p, stats);
     }
     AUR_LAUNCH_CHECK(ctx, "k_compat");
-  } else {
-    const aur_status st = pbr_passes(ctx, cam, o, s0, s1, stats, nullptr);
+  } else if (s1 > s0) {
+    const JobDev J = frame_job(ctx, cam, o);
+    PbrRun run;
+    run.jobs = &J;
+    run.njobs = 1;
+    run.begin = (unsigned long long)s0 * npix;
+    run.end = (unsigned long long)s1 * npix;
+    // whole samples per pass: enough paths in flight to fill the device
+    const uint32_t batch_cap = std::max<uint32_t>(1u, std::max<uint32_t>(ctx->render_params.max_paths, 1u) / npix);
+    run.cap = npix * std::min<uint32_t>(batch_cap, s1 - s0);
+    run.levels = ctx->levels;
+    run.nslots = ctx->num_slots;
+    run.max_depth = o->max_depth;
+    const aur_status st = pbr_passes<false>(ctx, run, stats, nullptr);
     if (st != AUR_OK) return st;
   }
   unsigned long long hs[2] = {0, 0};
@@ -880,6 +1102,7 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   if (nparams > kAdjMaxParams) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: more than 2048 shaders + lights");
   hipStream_t s = ctx->stream;
   const uint32_t npix = fr.width * fr.height;
+  if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: pixels * spp exceeds 2^32");
   AUR_HIP(ctx, fr.counters.reserve(8));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
@@ -894,7 +1117,18 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   k_adj_zero_u32<<<1, 64, 0, s>>>(adj.absmax);
   k_adj_absmax<<<std::min<uint32_t>(nblocks(npix, 256), 1024), 256, 0, s>>>(adj.dimg, npix, adj.absmax);
   AUR_LAUNCH_CHECK(ctx, "adjoint setup");
-  const aur_status st = pbr_passes(ctx, cam, o, 0, o->spp, stats, &adj);
+  const JobDev J = frame_job(ctx, cam, o);
+  PbrRun run;
+  run.jobs = &J;
+  run.njobs = 1;
+  run.begin = 0;
+  run.end = (unsigned long long)npix * o->spp;
+  const uint32_t batch_cap = std::max<uint32_t>(1u, std::max<uint32_t>(ctx->render_params.max_paths, 1u) / npix);
+  run.cap = npix * std::min<uint32_t>(batch_cap, o->spp);
+  run.levels = ctx->levels;
+  run.nslots = ctx->num_slots;
+  run.max_depth = o->max_depth;
+  const aur_status st = pbr_passes<false>(ctx, run, stats, &adj);
   if (st != AUR_OK) return st;
   k_adj_finish<<<nblocks(nparams, 256), 256, 0, s>>>(adj.total, 3u * ctx->num_shaders,
                                                      3u * ctx->num_lights, g_albedo, g_light);

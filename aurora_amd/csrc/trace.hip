@@ -41,8 +41,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                                         (uint64_t)gridDim.x * blockDim.x, dlo, true}
                             : lds_stack(stk_mem + threadIdx.x, blockDim.x);
   persistent_trace<kAny, kRefill>(
-      nodes, n_nodes, n, stk,
-      [&](uint64_t i, V3& o, V3& d, float& tmax) {
+      TreeRef{nodes, n_nodes}, n, stk,
+      [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef&) {
         const float4* r = reinterpret_cast<const float4*>(rays + i);
         const float4 a = r[0], b = r[1];
         o = mk(a.x, a.y, a.z);

@@ -237,15 +237,23 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   return !found;
 }
 
+// The tree a lane traverses: node records and their count.
+struct TreeRef {
+  const NodeRec* nodes;
+  uint32_t n_nodes;
+};
+
 // Persistent-wave traversal of a work range [0, n).  Wave w owns
 // [w*chunk, (w+1)*chunk); between node visits, once at least kRefill lanes
 // are idle (or all are), idle lanes take the next items of the range, ranked
-// by ballot (no atomics).  fetch(i, o, d, tmax) loads item i and returns false
-// if the item needs no traversal; store(i, s) receives the finished state.
+// by ballot (no atomics).  fetch(i, o, d, tmax, tree) loads item i, may
+// replace the lane's tree (initially tree0; batches of scenes), and returns
+// false if the item needs no traversal; store(i, s) receives the finished
+// state.  A fetch that never writes `tree` keeps the tree wave-uniform.
 template <bool kAny, int kRefill, class Fetch, class Store>
-__device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
-                                                 uint64_t n, const TravStack& stk, Fetch fetch,
-                                                 Store store) {
+__device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, const TravStack& stk,
+                                                 Fetch fetch, Store store) {
+  TreeRef tree = tree0;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t chunk = (n + waves - 1) / waves;
   const uint32_t lane = threadIdx.x & 63u;
@@ -266,9 +274,9 @@ __device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nod
           if (i < end) {
             V3 o, d;
             float tmax;
-            if (fetch(i, o, d, tmax)) {
+            if (fetch(i, o, d, tmax, tree)) {
               item = i;
-              active = trav_init<false>(s, nodes, o, d, tmax, nullptr);
+              active = trav_init<false>(s, tree.nodes, o, d, tmax, nullptr);
               if (!active) store(i, s);
             }
           }
@@ -278,7 +286,7 @@ __device__ __forceinline__ void persistent_trace(const NodeRec* __restrict__ nod
         break;
       }
     }
-    if (active && trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
+    if (active && trav_step<kAny, false>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
       store(item, s);
       active = false;
     }

@@ -55,8 +55,17 @@ class TraceParams(ctypes.Structure):
 
 
 class RenderParams(ctypes.Structure):
-    _fields_ = [("max_paths", ctypes.c_uint32)]
+    _fields_ = [("max_paths", ctypes.c_uint32), ("graph", ctypes.c_uint32)]
 
+
+# aur_job (include/aurora_amd.h): 8 x u32, u64 out_offset, aur_camera, padded to 8 B
+JOB_DTYPE = np.dtype({
+    "names": ["abi_version", "scene", "width", "height", "spp", "max_depth", "rr_start", "seed",
+              "out_offset", "camera"],
+    "formats": [np.uint32] * 8 + [np.uint64, scenes.CAMERA_DTYPE],
+    "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 40],
+    "itemsize": 104,
+})
 
 _lib = None
 
@@ -108,6 +117,10 @@ def load():
         "aur_frame_device_buffer": ([P, ctypes.POINTER(P)], i32),
         "aur_frame_rays_traced": ([P], u64),
         "aur_frame_copy": ([P, P], i32),
+        "aur_scene_add": ([P, u32, P, P, P, u32, P, u32, P, ctypes.POINTER(u32)], i32),
+        "aur_scene_clear": ([P], i32),
+        "aur_scene_count": ([P], u32),
+        "aur_batch_render": ([P, u32, P, P, P], i32),
         "aur_frame_adjoint": ([P, P, ctypes.POINTER(RenderOpts), P, P, P], i32),
     }
     for name, (args, res) in sig.items():
@@ -238,14 +251,14 @@ class Context:
         self._check(self._lib.aur_get_trace_params(self._h, ctypes.byref(p)), "get_trace_params")
         return {k: getattr(p, k) for k, _ in TraceParams._fields_}
 
-    def set_render_params(self, max_paths: int = 0) -> None:
-        p = RenderParams(max_paths)
+    def set_render_params(self, max_paths: int = 0, graph: int = 0) -> None:
+        p = RenderParams(max_paths, graph)
         self._check(self._lib.aur_set_render_params(self._h, ctypes.byref(p)), "set_render_params")
 
     def get_render_params(self) -> dict:
         p = RenderParams()
         self._check(self._lib.aur_get_render_params(self._h, ctypes.byref(p)), "get_render_params")
-        return {"max_paths": p.max_paths}
+        return {"max_paths": p.max_paths, "graph": p.graph}
 
     def trace_closest_stats_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> tuple[int, int]:
         p, q = ctypes.c_uint64(), ctypes.c_uint64()
@@ -312,6 +325,52 @@ class Context:
 
     def rays_traced(self) -> int:
         return int(self._lib.aur_frame_rays_traced(self._h))
+
+    # ---- batches ----
+    def scene_add(self, scene: scenes.Scene) -> int:
+        """Builds and keeps a scene for batches; returns its id."""
+        v = np.ascontiguousarray(scene.vertices, np.float32)
+        n = np.ascontiguousarray(scene.normals, np.float32)
+        ids = np.ascontiguousarray(scene.shader_ids, np.uint16)
+        sh = np.ascontiguousarray(scene.shaders)
+        li = np.ascontiguousarray(scene.lights)
+        out = ctypes.c_uint32()
+        self._check(self._lib.aur_scene_add(self._h, v.shape[0], _ptr(v), _ptr(n), _ptr(ids), sh.shape[0],
+                                            _ptr(sh), li.shape[0], _ptr(li) if li.shape[0] else None,
+                                            ctypes.byref(out)), "scene_add")
+        self.count = 0
+        return int(out.value)
+
+    def scene_clear(self) -> None:
+        self._check(self._lib.aur_scene_clear(self._h), "scene_clear")
+
+    def scene_count(self) -> int:
+        return int(self._lib.aur_scene_count(self._h))
+
+    @staticmethod
+    def make_jobs(specs) -> np.ndarray:
+        """specs: iterable of dicts with scene, width, height, spp, max_depth,
+        rr_start, seed, camera; out_offset is assigned densely in order."""
+        specs = list(specs)
+        jobs = np.zeros(len(specs), dtype=JOB_DTYPE)
+        off = 0
+        for i, sp in enumerate(specs):
+            for k in ("scene", "width", "height", "spp", "max_depth", "rr_start", "seed"):
+                jobs[i][k] = sp[k]
+            jobs[i]["abi_version"] = ABI_VERSION
+            jobs[i]["camera"] = sp["camera"]
+            jobs[i]["out_offset"] = off
+            off += int(sp["width"]) * int(sp["height"])
+        return jobs
+
+    def batch_render_ptr(self, jobs: np.ndarray, rgba_ptr: int, with_times: bool = False):
+        """Renders the jobs into the device buffer at rgba_ptr; returns the
+        per-job completion times (ms) when with_times."""
+        j = np.ascontiguousarray(jobs, dtype=JOB_DTYPE)
+        times = np.zeros(j.shape[0], np.float32) if with_times else None
+        self._check(self._lib.aur_batch_render(self._h, j.shape[0], _ptr(j), ctypes.c_void_p(rgba_ptr),
+                                               _ptr(times) if times is not None else None), "batch_render")
+        return times
 
     def frame_copy_ptr(self, rgba_ptr: int) -> None:
         """Copies the float4 accumulation buffer to device memory at rgba_ptr."""

@@ -174,12 +174,15 @@ aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
                             const aur_render_opts* opts);
 /* Path-loop launch shape (no reference counterpart: the reference traced
  * 65,536-ray tiles one sample at a time, src/render/monteCarloRaytracer.cpp:13).
- * max_paths: paths in flight per wavefront pass; a pass traces
- * clamp(max_paths / pixels, 1, remaining) samples of every pixel.  0 selects
- * the default (2^21).  Images do not depend on it: samples are accumulated in
- * sample order. */
+ * max_paths: paths in flight per wavefront pass; a frame pass traces
+ * clamp(max_paths / pixels, 1, remaining) samples of every pixel, a batch
+ * pass max_paths paths.  0 selects the default (2^21).
+ * graph: 1 captures one pass (camera, bounce loop, accumulation) in a HIP
+ * graph and replays it for every pass; 0 launches the kernels directly.
+ * Images do not depend on either: samples are accumulated in sample order. */
 typedef struct aur_render_params {
   uint32_t max_paths;
+  uint32_t graph;
 } aur_render_params;
 aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* params);
 aur_status aur_get_render_params(aur_context* ctx, aur_render_params* params);
@@ -220,6 +223,39 @@ uint64_t aur_frame_rays_traced(const aur_context* ctx);
  * Deterministic: the same inputs give the same bits. */
 aur_status aur_frame_adjoint(aur_context* ctx, const aur_camera* camera, const aur_render_opts* opts,
                              const float* d_image, float* grad_albedo, float* grad_radiance);
+
+/* ---- batches (no reference counterpart: the reference renders one frame
+ * of one scene per MonteCarloRaytracer::render call,
+ * src/render/monteCarloRaytracer.cpp:81-120) ----
+ * aur_scene_add builds a scene with the context's NMH builder and keeps it
+ * for batches; it replaces the context's current geometry, shaders and
+ * lights (upload again before aur_frame_render).  Arguments as for
+ * aur_geometry_upload / aur_shaders_upload / aur_lights_upload. */
+aur_status aur_scene_add(aur_context* ctx, uint32_t count, const float* vertices, const float* normals,
+                         const uint16_t* shader_ids, uint32_t num_shaders, const aur_shader* shaders,
+                         uint32_t num_lights, const aur_light* lights, uint32_t* scene_id);
+aur_status aur_scene_clear(aur_context* ctx);
+uint32_t aur_scene_count(const aur_context* ctx);
+
+/* One job: a PBR render (AUR_MODE_PBR semantics) of a kept scene. */
+typedef struct aur_job {
+  uint32_t abi_version; /* AUR_ABI_VERSION */
+  uint32_t scene;       /* id from aur_scene_add */
+  uint32_t width, height, spp, max_depth, rr_start, seed;
+  uint64_t out_offset;  /* first pixel of the job's image in rgba, in float4 units */
+  aur_camera camera;
+} aur_job;
+
+/* Renders every job; all jobs share each wavefront pass (render params:
+ * max_paths paths per pass, graph = 1 replays one captured pass).  rgba:
+ * device buffer, job i's float4 image (width*height, row 0 = bottom) at
+ * out_offset; images must not overlap.  Each image equals aur_frame_render
+ * of the same scene, camera and options bit for bit.  job_ms (optional,
+ * host, njobs floats): device time from the start of the batch to the end of
+ * the pass that completed each job (jobs are scheduled shortest first).
+ * Synchronous. */
+aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* jobs, float* rgba,
+                            float* job_ms);
 
 #ifdef __cplusplus
 }

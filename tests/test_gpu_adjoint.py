@@ -180,4 +180,4 @@ def test_adam_fit_reduces_loss(gpu_ctx):
             pr.clamp_(0.0, None)
         losses.append(float(loss.detach()))
     assert np.mean(losses[-5:]) < 0.5 * np.mean(losses[:3])
-    assert float((pr - gt_r).abs().mean()) < float((gt_r * 0.3).abs().mean())
+    assert float((pr.detach() - gt_r).abs().mean()) < float((gt_r * 0.3).abs().mean())
