@@ -7,6 +7,9 @@ Here launch shapes are picked by measurement on the device at hand:
 * tune_trace: persistent vs flat traversal, block size, refill threshold and
   resident waves per CU for aur_trace_closest/any, timed with HIP events on
   the context's stream over the caller's own rays.
+* tune_batch: paths in flight per wavefront pass (queue sizes) and direct
+  launches vs a replayed HIP graph for aur_batch_render, timed on a sample
+  of the caller's jobs.
 
 Results are cached in a JSON file keyed by device name and workload key, so a
 later process reuses them without re-measuring.  Results never depend on the
@@ -88,6 +91,44 @@ def tune_trace(ctx, n: int, rays_ptr: int, hits_ptr: int, key: str, reps: int = 
         table.append((dict(cand), time_trace(ctx, n, rays_ptr, hits_ptr, reps, any_hit)))
     best = min(table, key=lambda t: t[1])
     ctx.set_trace_params(**best[0])
+    if cache:
+        data[full_key] = {"params": best[0], "ms": best[1]}
+        _store(cache, data)
+    return best[0], table
+
+
+BATCH_CANDIDATES = tuple(dict(max_paths=1 << k, graph=g) for k in (20, 21, 22, 23) for g in (0, 1))
+
+
+def tune_batch(ctx, jobs, rgba_ptr: int, key: str, sample: int = 48, reps: int = 2,
+               cache: str | None = DEFAULT_CACHE, candidates=BATCH_CANDIDATES, force: bool = False):
+    """Sets the fastest render params for aur_batch_render on ctx and
+    returns (params, table).  Times `sample` jobs spread over the list (wall
+    time of the synchronous call, best of reps)."""
+    import time
+
+    import numpy as np
+
+    full_key = f"{_device_name()}|batch|{key}"
+    data = _load(cache) if cache else {}
+    if not force and full_key in data:
+        p = data[full_key]["params"]
+        ctx.set_render_params(**p)
+        return p, []
+    idx = np.linspace(0, len(jobs) - 1, min(sample, len(jobs))).astype(int)
+    sub = jobs[np.unique(idx)].copy()
+    table = []
+    for cand in candidates:
+        ctx.set_render_params(**cand)
+        ctx.batch_render_ptr(sub, rgba_ptr)  # warm-up: allocations, graph path
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.batch_render_ptr(sub, rgba_ptr)
+            best = min(best, time.perf_counter() - t0)
+        table.append((dict(cand), best * 1e3))
+    best = min(table, key=lambda t: t[1])
+    ctx.set_render_params(**best[0])
     if cache:
         data[full_key] = {"params": best[0], "ms": best[1]}
         _store(cache, data)

@@ -463,9 +463,139 @@ def workload_adjoint_c5(a) -> dict:
     }
 
 
+def c4_jobs(n_jobs: int = 256, seed: int = 44):
+    """256 preview jobs: Cornell scenes with 2..40 subdivision-4 spheres
+    (10k-205k triangles), square sides in {64,128,256,384,512}, spp in
+    {4,16,64}, depth 6, Russian roulette from bounce 3."""
+    from aurora_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_jobs):
+        side = int(rng.choice([64, 128, 256, 384, 512]))
+        spp = int(rng.choice([4, 16, 64]))
+        n_sph = int(rng.integers(2, 41))
+        s = scenes.cornell(seed=1000 + k, n_spheres=n_sph, subdivisions=4, width=side, height=side,
+                           spp=spp, max_depth=6)
+        out.append(s)
+    return out
+
+
+def workload_batch_c4(a) -> dict:
+    """256 jobs in shared wavefront passes (aur_batch_render) vs the same
+    jobs rendered one by one with aur_frame_render (scenes built beforehand
+    in both cases).  Images are compared bit for bit, so both sides reach the
+    same rel-RMSE."""
+    import torch
+
+    from aurora_amd import native, tune
+
+    t0 = time.perf_counter()
+    sc = c4_jobs()
+    tri = np.array([s.count for s in sc])
+    paths = np.array([s.width * s.height * s.spp for s in sc], dtype=np.int64)
+    log(f"c4: {len(sc)} jobs, {tri.min()}..{tri.max()} triangles, {paths.sum() / 1e6:.0f}M paths "
+        f"({time.perf_counter() - t0:.1f}s)")
+    order = np.argsort(paths, kind="stable")  # both sides run shortest jobs first
+
+    # per-job launches: one frame render per job, (A) with the frame
+    # renderer's multi-sample passes, (B) one sample per pass, the
+    # reference's launch structure (src/render/monteCarloRaytracer.cpp:81-120)
+    if a.batch_only:  # profiling the batched passes alone
+        order = order[:0]
+    single = native.Context(0)
+    t_single = np.zeros((2, len(sc)))
+    rays_single = 0
+    images = {}
+    check = set(range(len(sc)))  # every image is compared
+    for k in order:
+        s = sc[k]
+        single.upload_scene(s)
+        single.rebuild_nmh()
+        single.frame_create(s.width, s.height, s.spp)
+        o = native.Context.opts(native.MODE_PBR, s.spp, s.max_depth, s.rr_start, 7 + int(k))
+        for mode, max_paths in ((0, 0), (1, 1)):
+            single.set_render_params(max_paths)
+            if k == order[0]:
+                single.frame_render(s.camera, o)  # warm-up
+            t1 = time.perf_counter()
+            single.frame_render(s.camera, o)
+            t_single[mode, k] = time.perf_counter() - t1
+        rays_single += single.rays_traced()
+        if k in check:
+            images[int(k)] = single.frame_read()
+    single.close()
+    T_single = float(t_single[0].sum()) or float("nan")
+    T_plain = float(t_single[1].sum()) or float("nan")
+    # all jobs queued at t=0, shortest first
+    lat_single = np.cumsum(t_single[0][order]) if len(order) else np.array([np.nan])
+    lat_plain = np.cumsum(t_single[1][order]) if len(order) else np.array([np.nan])
+    log(f"per-job (A, multi-sample passes): {T_single:.3f}s total, {len(sc) / T_single:.1f} jobs/s, "
+        f"{rays_single / T_single / 1e6:.0f} Mrays/s")
+    log(f"per-job (B, one sample per pass): {T_plain:.3f}s total, {len(sc) / T_plain:.1f} jobs/s, "
+        f"{rays_single / T_plain / 1e6:.0f} Mrays/s")
+
+    # batched
+    ctx = native.Context(0)
+    ids = [ctx.scene_add(s) for s in sc]
+    jobs = native.Context.make_jobs(
+        dict(scene=ids[k], width=s.width, height=s.height, spp=s.spp, max_depth=s.max_depth,
+             rr_start=s.rr_start, seed=7 + k, camera=s.camera) for k, s in enumerate(sc))
+    total_px = int(sum(s.width * s.height for s in sc))
+    out = torch.zeros((total_px, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    params, table = tune.tune_batch(ctx, jobs, out.data_ptr(), key="c4", force=a.tune)
+    log(f"batch params {params}" + (f" (tuned: {[(p['max_paths'], p['graph'], round(ms, 1)) for p, ms in table]})"
+                                    if table else " (cached)"))
+    ctx.batch_render_ptr(jobs, out.data_ptr())  # warm-up
+    runs = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        job_ms = ctx.batch_render_ptr(jobs, out.data_ptr(), with_times=True)
+        runs.append((time.perf_counter() - t1, job_ms))
+    T_batch, job_ms = min(runs, key=lambda r: r[0])
+    rays_batch = ctx.rays_traced()
+    log(f"batched: {T_batch:.3f}s, {len(sc) / T_batch:.1f} jobs/s, {rays_batch / T_batch / 1e6:.0f} Mrays/s")
+    img = out.cpu().numpy()
+    same = 0
+    for k in check if not a.batch_only else ():
+        o0 = int(jobs[k]["out_offset"])
+        n = sc[k].width * sc[k].height
+        same += int(np.array_equal(img[o0:o0 + n], images[k]))
+    log(f"identical images: {same}/{len(check)}")
+    ctx.close()
+    lat_batch = np.asarray(job_ms, dtype=np.float64) / 1e3
+    speedup = T_single / T_batch
+    return {
+        "metric": "batched jobs/s (c4, 256 jobs)", "value": round(len(sc) / T_batch, 2), "unit": "jobs/s",
+        "n_gpus": 1, "steps": 3, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (256 seeded Cornell scenes, 2-40 spheres)",
+        "config": {
+            "workload": "c4: 256 jobs, sides {64..512}, spp {4,16,64}, depth 6, RR from bounce 3",
+            "triangles_min_max": [int(tri.min()), int(tri.max())], "paths_total": int(paths.sum()),
+            "batch_s": round(T_batch, 4),
+            "per_job_multisample_s": round(T_single, 4),
+            "per_job_one_sample_per_pass_s": round(T_plain, 4),
+            "speedup_vs_per_job_multisample": round(speedup, 3),
+            "speedup_vs_per_job_one_sample_per_pass": round(T_plain / T_batch, 3),
+            "per_job_jobs_per_s": [round(len(sc) / T_single, 2), round(len(sc) / T_plain, 2)],
+            "batch_latency_p50_p99_s": [round(float(np.percentile(lat_batch, 50)), 4),
+                                        round(float(np.percentile(lat_batch, 99)), 4)],
+            "per_job_multisample_latency_p50_p99_s": [round(float(np.percentile(lat_single, 50)), 4),
+                                                      round(float(np.percentile(lat_single, 99)), 4)],
+            "per_job_one_sample_per_pass_latency_p50_p99_s": [
+                round(float(np.percentile(lat_plain, 50)), 4), round(float(np.percentile(lat_plain, 99)), 4)],
+            "batch_mrays": round(rays_batch / T_batch / 1e6, 1),
+            "per_job_mrays": round(rays_single / T_single / 1e6, 1),
+            "identical_images": f"{same}/{len(check)}",
+            "render_params": params,
+        },
+    }
+
+
 WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m,
              "render_c1": workload_render_c1, "render_c2": workload_render_c2,
-             "adjoint_c5": workload_adjoint_c5}
+             "adjoint_c5": workload_adjoint_c5, "batch_c4": workload_batch_c4}
 
 
 def main() -> None:
@@ -479,6 +609,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
     ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
+    ap.add_argument("--batch-only", action="store_true", help="batch_c4: skip the per-job side")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
