@@ -141,6 +141,8 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
   AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
   AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   fr.rays_traced = hs[0] + hs[1];
+  fr.rays_extension = hs[0];
+  fr.rays_shadow = hs[1];
   if (job_ms) {
     // a job is complete at the end of the pass holding its last path
     for (uint32_t k = 0; k < njobs; ++k) {

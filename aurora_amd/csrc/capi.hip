@@ -323,6 +323,13 @@ aur_status aur_frame_copy(aur_context* ctx, float* rgba) {
 
 uint64_t aur_frame_rays_traced(const aur_context* ctx) { return ctx ? ctx->frame.rays_traced : 0; }
 
+aur_status aur_frame_ray_counts(const aur_context* ctx, uint64_t* camera_and_extension, uint64_t* shadow) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (camera_and_extension) *camera_and_extension = ctx->frame.rays_extension;
+  if (shadow) *shadow = ctx->frame.rays_shadow;
+  return AUR_OK;
+}
+
 aur_status aur_frame_adjoint(aur_context* ctx, const aur_camera* camera, const aur_render_opts* opts,
                              const float* d_image, float* grad_albedo, float* grad_radiance) {
   if (!ctx) return AUR_ERR_INVALID_ARG;

@@ -103,6 +103,7 @@ struct FrameState {
   DevBuf<unsigned long long> pass_g0;
   size_t shadow_cap = 0;
   uint64_t rays_traced = 0;
+  uint64_t rays_extension = 0, rays_shadow = 0;  // of the last render
 };
 
 }  // namespace aur

@@ -35,6 +35,7 @@ namespace aur {
 namespace {
 
 constexpr uint32_t kTile = 2048;  // radix / emit tile (elements)
+constexpr uint32_t kChunk = 32;   // tiles per chunk of the radix offset scan
 constexpr uint32_t kSubT = 2048;  // max elements of an LDS-resident subtree
 constexpr int kThreads = 256;
 
@@ -45,7 +46,11 @@ struct ChildDesc {
   uint32_t off, size, tb, te;
 };
 struct GroupDesc {
-  uint32_t off, size, tb, te;
+  uint32_t off, size, tb, te, cb, ce;  // tiles [tb, te), offset chunks [cb, ce)
+};
+// Up to kChunk consecutive tiles of one group: the unit of the offset scan.
+struct ChunkDesc {
+  uint32_t tb, te;
 };
 struct SubGroup {
   uint32_t off, size, first, count;
@@ -117,15 +122,28 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restr
   H[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
-// Per group: digit-major, tile-minor exclusive offsets (stable scatter).
-__global__ __launch_bounds__(kThreads) void k_radix_offsets(const GroupDesc* __restrict__ groups,
-                                                            const uint32_t* __restrict__ H,
-                                                            uint32_t* __restrict__ O) {
+// Digit-major, tile-minor exclusive offsets of every tile (stable scatter),
+// as a reduce-then-scan over chunks of tiles so that no thread walks a whole
+// group: chunk sums, one scan per group over its chunks, then the tiles of
+// each chunk.
+__global__ __launch_bounds__(kThreads) void k_radix_chunk_sums(const ChunkDesc* __restrict__ chunks,
+                                                               const uint32_t* __restrict__ H,
+                                                               uint32_t* __restrict__ S) {
+  const ChunkDesc c = chunks[blockIdx.x];
+  const uint32_t d = threadIdx.x;
+  uint32_t sum = 0;
+  for (uint32_t t = c.tb; t < c.te; ++t) sum += H[(size_t)t * 256 + d];
+  S[(size_t)blockIdx.x * 256 + d] = sum;
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* __restrict__ groups,
+                                                               const uint32_t* __restrict__ S,
+                                                               uint32_t* __restrict__ B) {
   __shared__ uint32_t s[256];
   const GroupDesc g = groups[blockIdx.x];
   const uint32_t d = threadIdx.x;
   uint32_t total = 0;
-  for (uint32_t t = g.tb; t < g.te; ++t) total += H[(size_t)t * 256 + d];
+  for (uint32_t c = g.cb; c < g.ce; ++c) total += S[(size_t)c * 256 + d];
   s[d] = total;
   __syncthreads();
   for (uint32_t off = 1; off < 256; off <<= 1) {
@@ -135,7 +153,20 @@ __global__ __launch_bounds__(kThreads) void k_radix_offsets(const GroupDesc* __r
     __syncthreads();
   }
   uint32_t run = g.off + s[d] - total;
-  for (uint32_t t = g.tb; t < g.te; ++t) {
+  for (uint32_t c = g.cb; c < g.ce; ++c) {
+    B[(size_t)c * 256 + d] = run;
+    run += S[(size_t)c * 256 + d];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const ChunkDesc* __restrict__ chunks,
+                                                                  const uint32_t* __restrict__ H,
+                                                                  const uint32_t* __restrict__ B,
+                                                                  uint32_t* __restrict__ O) {
+  const ChunkDesc c = chunks[blockIdx.x];
+  const uint32_t d = threadIdx.x;
+  uint32_t run = B[(size_t)blockIdx.x * 256 + d];
+  for (uint32_t t = c.tb; t < c.te; ++t) {
     O[(size_t)t * 256 + d] = run;
     run += H[(size_t)t * 256 + d];
   }
@@ -472,8 +503,8 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 
 struct LevelPlan {
   int level;
-  uint32_t m, first, nchildren, ngroups, ntiles;
-  size_t child_base, group_base, tile_base;
+  uint32_t m, first, nchildren, ngroups, ntiles, nchunks;
+  size_t child_base, group_base, tile_base, chunk_base;
 };
 
 }  // namespace
@@ -514,8 +545,9 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   std::vector<TileDesc> tiles;
   std::vector<ChildDesc> children;
   std::vector<GroupDesc> groups;
+  std::vector<ChunkDesc> chunks;
   std::vector<LevelPlan> plan;
-  size_t max_tiles = 1, max_children = 1;
+  size_t max_tiles = 1, max_children = 1, max_chunks = 1;
   for (int l = 0; l < level0; ++l) {
     LevelPlan lp{};
     lp.level = l;
@@ -532,7 +564,7 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
       const int64_t parent = l == 0 ? 0 : (int64_t)((h - 1) / 2);
       if (parent != cur_parent) {
         if (!groups.empty() && groups.size() > lp.group_base) groups.back().te = (uint32_t)(tiles.size() - lp.tile_base);
-        groups.push_back(GroupDesc{(uint32_t)off, 0, (uint32_t)(tiles.size() - lp.tile_base), 0});
+        groups.push_back(GroupDesc{(uint32_t)off, 0, (uint32_t)(tiles.size() - lp.tile_base), 0, 0, 0});
         cur_parent = parent;
       }
       GroupDesc& g = groups.back();
@@ -550,6 +582,15 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     lp.nchildren = (uint32_t)(children.size() - lp.child_base);
     lp.ngroups = (uint32_t)(groups.size() - lp.group_base);
     lp.ntiles = (uint32_t)(tiles.size() - lp.tile_base);
+    lp.chunk_base = chunks.size();
+    for (size_t gi = lp.group_base; gi < groups.size(); ++gi) {
+      GroupDesc& g = groups[gi];
+      g.cb = (uint32_t)(chunks.size() - lp.chunk_base);
+      for (uint32_t t = g.tb; t < g.te; t += kChunk) chunks.push_back(ChunkDesc{t, std::min(t + kChunk, g.te)});
+      g.ce = (uint32_t)(chunks.size() - lp.chunk_base);
+    }
+    lp.nchunks = (uint32_t)(chunks.size() - lp.chunk_base);
+    max_chunks = std::max<size_t>(max_chunks, lp.nchunks);
     max_tiles = std::max<size_t>(max_tiles, lp.ntiles);
     max_children = std::max<size_t>(max_children, lp.nchildren);
     plan.push_back(lp);
@@ -573,11 +614,12 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
 
   // ---- device buffers ----
   DevBuf<float> bmin, bmax;
-  DevBuf<uint32_t> Wa, Wb, Ka, Kb, H, O, argpos;
+  DevBuf<uint32_t> Wa, Wb, Ka, Kb, H, O, CS, CB, argpos;
   DevBuf<uint2> partial;
   DevBuf<TileDesc> dtiles;
   DevBuf<ChildDesc> dchildren;
   DevBuf<GroupDesc> dgroups;
+  DevBuf<ChunkDesc> dchunks;
   DevBuf<SubGroup> dsub;
   AUR_HIP(ctx, bmin.reserve((size_t)3 * N));
   AUR_HIP(ctx, bmax.reserve((size_t)3 * N));
@@ -589,6 +631,10 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     AUR_HIP(ctx, Kb.reserve(N));
     AUR_HIP(ctx, H.reserve(max_tiles * 256));
     AUR_HIP(ctx, O.reserve(max_tiles * 256));
+    AUR_HIP(ctx, CS.reserve(max_chunks * 256));
+    AUR_HIP(ctx, CB.reserve(max_chunks * 256));
+    AUR_HIP(ctx, dchunks.reserve(chunks.size()));
+    AUR_HIP(ctx, hipMemcpy(dchunks.p, chunks.data(), chunks.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice));
     AUR_HIP(ctx, partial.reserve(max_tiles));
     AUR_HIP(ctx, argpos.reserve(max_children));
     AUR_HIP(ctx, dtiles.reserve(tiles.size()));
@@ -618,12 +664,15 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     const TileDesc* tl = dtiles.p + lp.tile_base;
     const ChildDesc* ch = dchildren.p + lp.child_base;
     const GroupDesc* gr = dgroups.p + lp.group_base;
+    const ChunkDesc* ck = dchunks.p + lp.chunk_base;
     k_gather_keys<<<blocks(lp.m, 256), 256, 0, s>>>(Wcur, lp.m, N, bmin.p + (size_t)a * N, Ka.p);
     uint32_t *kin = Ka.p, *kout = Kb.p, *vin = Wcur, *vout = Walt;
     for (int pass = 0; pass < 4; ++pass) {
       const uint32_t shift = 8u * pass;
       k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, shift, H.p);
-      k_radix_offsets<<<lp.ngroups, kThreads, 0, s>>>(gr, H.p, O.p);
+      k_radix_chunk_sums<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CS.p);
+      k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, CS.p, CB.p);
+      k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CB.p, O.p);
       k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, O.p, lp.m, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);

@@ -1084,6 +1084,8 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
   AUR_HIP(ctx, hipStreamSynchronize(s));
   fr.rays_traced = hs[0] + hs[1];
+  fr.rays_extension = hs[0];
+  fr.rays_shadow = hs[1];
   return AUR_OK;
 }
 
@@ -1137,6 +1139,8 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   AUR_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
   AUR_HIP(ctx, hipStreamSynchronize(s));
   fr.rays_traced = hs[0] + hs[1];
+  fr.rays_extension = hs[0];
+  fr.rays_shadow = hs[1];
   return AUR_OK;
 }
 

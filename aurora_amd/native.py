@@ -116,6 +116,7 @@ def load():
         "aur_frame_read": ([P, P], i32),
         "aur_frame_device_buffer": ([P, ctypes.POINTER(P)], i32),
         "aur_frame_rays_traced": ([P], u64),
+        "aur_frame_ray_counts": ([P, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
         "aur_frame_copy": ([P, P], i32),
         "aur_scene_add": ([P, u32, P, P, P, u32, P, u32, P, ctypes.POINTER(u32)], i32),
         "aur_scene_clear": ([P], i32),
@@ -325,6 +326,12 @@ class Context:
 
     def rays_traced(self) -> int:
         return int(self._lib.aur_frame_rays_traced(self._h))
+
+    def ray_counts(self) -> tuple[int, int]:
+        """(camera + extension rays, shadow rays) of the last render."""
+        e, sh = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._lib.aur_frame_ray_counts(self._h, ctypes.byref(e), ctypes.byref(sh)), "ray_counts")
+        return int(e.value), int(sh.value)
 
     # ---- batches ----
     def scene_add(self, scene: scenes.Scene) -> int:

@@ -210,15 +210,30 @@ def workload_soup10m(a) -> dict:
                "kind": "port",
                "sample": "literal NMH build of the first 2,097,152 soup triangles, one core"}
     ctx.close()
+    build_ms = float(np.median(builds[1:]))
+    nb = nmh_build_bytes(scene.count)
+    achieved = nb / (build_ms * 1e-3) / 1e9
     return {
-        "metric": "NMH build ms @10M tri", "value": round(float(np.median(builds[1:])), 3),
+        "metric": "NMH build ms @10M tri", "value": round(build_ms, 3),
         "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded 10M-triangle soup)",
         "config": {"workload": "10M-triangle soup build + 2^24 incoherent + 4096^2 coherent rays",
                    "triangles": scene.count, "incoherent_mrays": out["incoherent"],
-                   "coherent_mrays": out["coherent"]},
+                   "coherent_mrays": out["coherent"], "build_ms_all": [round(b, 3) for b in builds]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": nb},
         "cpu_baseline": cpu,
     }
+
+
+def nmh_build_bytes(n: int) -> int:
+    """Algorithmic bytes of one NMH build of n triangles:
+    60 n (bounds) + sum_{l<L} 12 P_l (one stable pass per level) +
+    sum_{l<L-1} 8 P_l (argmax) + 196 n (final gather), P_l = n - 2(2^l - 1)."""
+    levels = (n // 2).bit_length()  # log2i(n/2) + 1
+    pend = [n - 2 * ((1 << l) - 1) for l in range(levels)]
+    return 60 * n + 12 * sum(pend) + 8 * sum(pend[:-1]) + 196 * n
 
 
 def two_seed_rel_rmse(img_a: np.ndarray, img_b: np.ndarray) -> float:
@@ -262,12 +277,14 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
         img_a = ctx.frame_read()
         dt = time.perf_counter() - t0
         rays = ctx.rays_traced()
+        rays_ext, rays_sh = ctx.ray_counts()
         ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, spp, scene.max_depth,
                                                            scene.rr_start, 8))
         img_b = ctx.frame_read()
         err = two_seed_rel_rmse(img_a, img_b)
         rows.append({"spp": spp, "s": round(dt, 4), "rel_rmse": round(err, 5),
-                     "mrays": round(rays / dt / 1e6, 1), "rays": rays})
+                     "mrays": round(rays / dt / 1e6, 1), "rays": rays, "extension_rays": rays_ext,
+                     "shadow_rays": rays_sh})
         log(f"{label}: {spp} spp {dt:.3f}s rel-RMSE {err:.4f} {rays / dt / 1e6:.0f} Mrays/s")
         if err <= 0.01:
             hit = rows[-1]

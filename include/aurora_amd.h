@@ -204,6 +204,9 @@ aur_status aur_frame_device_buffer(aur_context* ctx, float** rgba);
 /* Number of ray segments (camera + extension + shadow) traced by the last
  * aur_frame_render / aur_frame_render_samples call. */
 uint64_t aur_frame_rays_traced(const aur_context* ctx);
+/* The same count split into camera + extension (closest hit) and shadow
+ * (any hit) rays. */
+aur_status aur_frame_ray_counts(const aur_context* ctx, uint64_t* camera_and_extension, uint64_t* shadow);
 
 /* ---- adjoint (no reference counterpart: the reference renders only; this
  * is the differentiable form of the AUR_MODE_PBR path loop) ----
