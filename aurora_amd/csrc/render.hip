@@ -448,11 +448,11 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   const JobDev& J = job_of<kMulti>(a, p);
   const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
   const uint32_t pix = r0 % J.npix, smp = r0 / J.npix;
-  const size_t base = (size_t)j * a.nslots;
+  const size_t base = j, ss = a.cap;  // slot e of queue entry j at base + e * ss
   const size_t np = a.cap;
   const aur_hit hit = a.hits[p];
   if (hit.id < 0) {
-    for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s].w = 0.0f;
+    for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s * ss].w = 0.0f;
     if (kRecord) a.rec_mat[bounce * np + p] = -1;
     return;
   }
@@ -492,11 +492,11 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
           const float wgt = (cosS * cosL * Lt.area) / (dist * dist) * inv;
           const V3 c = bf * (CI * wgt);
           const V3 so = P + (kEpsilon * wi);
-          a.sh_o[base + e] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
-          a.sh_c[base + e] = make_float4(c.x, c.y, c.z, wgt * kInvPi);
+          a.sh_o[base + e * ss] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
+          a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, wgt * kInvPi);
           flag = 1.0f;
         }
-        a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+        a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       }
     } else if (Lt.type == AUR_LIGHT_POINT) {
       V3 wi = ld3(Lt.position) - P;
@@ -506,11 +506,11 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
       float flag = 0.0f;
       if (cosS > 0.0f) {
         const V3 c = bf * ((CI / (dist * dist)) * cosS);
-        a.sh_o[base + e] = make_float4(P.x, P.y, P.z, dist);
-        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, cosS / (dist * dist) * kInvPi);
+        a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, dist);
+        a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS / (dist * dist) * kInvPi);
         flag = 1.0f;
       }
-      a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+      a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
     } else if (Lt.type == AUR_LIGHT_DIRECTIONAL) {
       const V3 wi = -ld3(Lt.direction);
@@ -518,21 +518,21 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
       float flag = 0.0f;
       if (cosS > 0.0f) {
         const V3 c = bf * (CI * cosS);
-        a.sh_o[base + e] = make_float4(P.x, P.y, P.z, kInfinity);
-        a.sh_c[base + e] = make_float4(c.x, c.y, c.z, cosS * kInvPi);
+        a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, kInfinity);
+        a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS * kInvPi);
         flag = 1.0f;
       }
-      a.sh_d[base + e] = make_float4(wi.x, wi.y, wi.z, flag);
+      a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
     } else {
       const V3 c = bf * (CI * Lt.ambient);
-      a.sh_c[base + e] = make_float4(c.x, c.y, c.z, Lt.ambient * kInvPi);
-      a.sh_d[base + e] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);  // no shadow ray
+      a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, Lt.ambient * kInvPi);
+      a.sh_d[base + e * ss] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);  // no shadow ray
       ++e;
     }
   }
   if constexpr (kMulti) {
-    for (; e < a.nslots; ++e) a.sh_d[base + e].w = 0.0f;  // slots of jobs with more lights
+    for (; e < a.nslots; ++e) a.sh_d[base + e * ss].w = 0.0f;  // slots of jobs with more lights
   }
   if (bounce + 1 >= J.max_depth) return;
   const U4 r = pbr_draw(J.seed, pix, smp, bounce, 1);
@@ -572,15 +572,19 @@ template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
                                                        const uint32_t* __restrict__ qin) {
   extern __shared__ float2 stk_mem[];
-  const uint64_t total = (uint64_t)a.counters[cin] * a.nslots;
+  // items slot-major: one wave traces neighbouring paths' rays to the same light
+  const uint32_t n_in = a.counters[cin];
+  const uint64_t total = (uint64_t)n_in * a.nslots;
   uint32_t traced = 0;
   persistent_trace<true, kRefill>(
       TreeRef{a.job.nodes, a.job.n_nodes}, total, lds_stack(stk_mem + threadIdx.x, blockDim.x),
-      [&](uint64_t e, V3& o, V3& d, float& tmax, TreeRef& tree) {
+      [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree) {
+        const uint32_t sl = (uint32_t)(i / n_in), j = (uint32_t)(i - (uint64_t)sl * n_in);
+        const size_t e = (size_t)sl * a.cap + j;
         const float4 d4 = a.sh_d[e];
         if (d4.w != 1.0f) return false;
         if constexpr (kMulti) {
-          const JobDev& J = a.jobs[a.path_job[qin[e / a.nslots]]];
+          const JobDev& J = a.jobs[a.path_job[qin[j]]];
           tree = TreeRef{J.nodes, J.n_nodes};
         }
         const float4 o4 = a.sh_o[e];
@@ -590,7 +594,10 @@ __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
         ++traced;
         return true;
       },
-      [&](uint64_t e, const Trav& s) { a.sh_occ[e] = s.id >= 0 ? 1 : 0; });
+      [&](uint64_t i, const Trav& s) {
+        const uint32_t sl = (uint32_t)(i / n_in), j = (uint32_t)(i - (uint64_t)sl * n_in);
+        a.sh_occ[(size_t)sl * a.cap + j] = s.id >= 0 ? 1 : 0;
+      });
   count_add(&a.stats[1], traced);
 }
 
@@ -600,16 +607,16 @@ __global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_
   if (j >= a.counters[cin]) return;
   const uint32_t p = qin[j];
   if (a.hits[p].id < 0) return;
-  const size_t base = (size_t)j * a.nslots;
   const size_t np = a.cap;
   float4 L4 = a.radiance[p];
   V3 L = mk(L4.x, L4.y, L4.z);
   for (uint32_t s = 0; s < a.nslots; ++s) {
-    const float flag = a.sh_d[base + s].w;
-    const bool lit = flag == 2.0f || (flag == 1.0f && a.sh_occ[base + s] == 0);
+    const size_t e = (size_t)s * a.cap + j;
+    const float flag = a.sh_d[e].w;
+    const bool lit = flag == 2.0f || (flag == 1.0f && a.sh_occ[e] == 0);
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (lit) {
-      c = a.sh_c[base + s];
+      c = a.sh_c[e];
       L = L + mk(c.x, c.y, c.z);
     }
     if (kRecord) a.rec_w[((size_t)bounce * a.nslots + s) * np + p] = c.w;
