@@ -160,8 +160,12 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     cnt->passed += 1;
     cnt->popped += (2 * k + 1 < n_nodes) + (2 * k + 2 < n_nodes);
   }
+  // the whole record in one round of loads (the right child's slab too, used
+  // only when the node has two children: a load issued inside that branch
+  // would wait a second memory latency per visit)
   const float4* rec = reinterpret_cast<const float4*>(nodes + k);
   const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
+  const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
   float u, v, t;
   bool stop = false;  // any hit found
   if (tri_test(s.o, s.d, mk(a0.x, a0.y, a0.z), mk(a0.w, a1.x, a1.y), mk(a1.z, a1.w, a2.x), u, v, t) &&
@@ -195,7 +199,6 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     float2 rr = make_float2(0.0f, 0.0f);
     bool okr = false;
     if (cl + 1 < n_nodes) {
-      const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
       rr = narrow(s.r, cs.x, cs.y, oc, dc, yc, fd);
       okr = rr.x <= rr.y;
     }
@@ -250,6 +253,8 @@ struct TreeRef {
 // replace the lane's tree (initially tree0; batches of scenes), and returns
 // false if the item needs no traversal; store(i, s) receives the finished
 // state.  A fetch that never writes `tree` keeps the tree wave-uniform.
+// (Splitting a refill over two iterations, so that the item loads overlap
+// the active lanes' node step, measured slower: the new lanes lose a step.)
 template <bool kAny, int kRefill, class Fetch, class Store>
 __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, const TravStack& stk,
                                                  Fetch fetch, Store store) {
