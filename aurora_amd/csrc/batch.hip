@@ -118,6 +118,9 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
     J.rr_start = j.rr_start;
     J.seed = j.seed;
     J.paths = J.npix * j.spp;
+    J.rx0 = 0;
+    J.ry0 = 0;
+    J.rw = j.width;
     J.cam = j.camera;
     start += J.paths;
     run.levels = std::max(run.levels, sc.levels);

@@ -253,7 +253,50 @@ aur_status aur_frame_create(aur_context* ctx, uint32_t width, uint32_t height, u
   fr.width = width;
   fr.height = height;
   fr.spp = spp;
+  fr.left = fr.bottom = 0;
+  fr.right = width - 1;
+  fr.top = height - 1;
   return aur_frame_clear(ctx);
+}
+
+aur_status aur_frame_set_region(aur_context* ctx, uint32_t left, uint32_t right, uint32_t bottom, uint32_t top) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  aur::FrameState& fr = ctx->frame;
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "region: no frame");
+  if (left > right || bottom > top || right >= fr.width || top >= fr.height)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "region: outside the frame or empty");
+  fr.left = left;
+  fr.right = right;
+  fr.bottom = bottom;
+  fr.top = top;
+  return AUR_OK;
+}
+
+aur_status aur_frame_get_region(aur_context* ctx, uint32_t* left, uint32_t* right, uint32_t* bottom, uint32_t* top) {
+  if (!ctx || !left || !right || !bottom || !top) return AUR_ERR_INVALID_ARG;
+  const aur::FrameState& fr = ctx->frame;
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "region: no frame");
+  *left = fr.left;
+  *right = fr.right;
+  *bottom = fr.bottom;
+  *top = fr.top;
+  return AUR_OK;
+}
+
+aur_status aur_frame_filter(aur_context* ctx) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  return aur::frame_filter(ctx);
+}
+
+aur_status aur_frame_write(aur_context* ctx, const float* rgba) {
+  if (!ctx || !rgba) return AUR_ERR_INVALID_ARG;
+  aur::FrameState& fr = ctx->frame;
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "frame: none");
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  AUR_HIP(ctx, hipMemcpy(fr.pixels.p, rgba, sizeof(float4) * fr.width * fr.height, hipMemcpyHostToDevice));
+  return AUR_OK;
 }
 
 aur_status aur_frame_destroy(aur_context* ctx) {

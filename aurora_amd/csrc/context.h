@@ -56,8 +56,9 @@ constexpr uint32_t kDefaultMaxPaths = 1u << 21;
 // One render job of the PBR wavefront: a camera over one built scene.  A
 // frame render is a single job; a batch is many.  Paths are numbered
 // globally: the job owns [path_start, path_start + paths), sample-major, so
-// global path g is pixel (g - path_start) % npix of sample
-// (g - path_start) / npix.
+// global path g is region pixel (g - path_start) % npix of sample
+// (g - path_start) / npix.  Region pixel l is frame pixel
+// (ry0 + l / rw) * w + rx0 + l % rw (the whole frame: rx0 = ry0 = 0, rw = w).
 struct JobDev {
   const NodeRec* nodes;
   const float* normals;  // tree order, 9 floats per triangle
@@ -68,6 +69,7 @@ struct JobDev {
   unsigned long long path_start;
   uint32_t n_nodes, num_shaders, num_lights, nslots;
   uint32_t w, h, npix, spp, max_depth, rr_start, seed, paths;
+  uint32_t rx0, ry0, rw;
   aur_camera cam;
 };
 
@@ -83,6 +85,8 @@ struct SceneStore {
 
 struct FrameState {
   uint32_t width = 0, height = 0, spp = 0;
+  // render region, inclusive (Rect left/right/bottom/top, src/util/math.h)
+  uint32_t left = 0, right = 0, bottom = 0, top = 0;
   DevBuf<float4> pixels;  // RGBA accumulation (framebuffer)
   // PBR wavefront state, indexed by path (pixel + pixels * sample-in-pass)
   DevBuf<float4> ray_o, ray_d, beta, radiance;
@@ -180,6 +184,7 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
                         uint32_t s0, uint32_t s1);
 aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
                          const float* d_image, float* g_albedo, float* g_light);
+aur_status frame_filter(aur_context* ctx);
 
 // PBR wavefront passes over the global path range [begin, end) of `jobs`
 // (host array; one job renders with wave-uniform parameters, several share

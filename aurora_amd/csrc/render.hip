@@ -105,17 +105,21 @@ __device__ inline bool occluded(const SceneDev& sc, V3 o, V3 d, float tmax, floa
   return trace_ray<true>(sc.nodes, sc.n_nodes, o, d, tmax, stk, stride).id >= 0;
 }
 
+// region: (left, bottom, region width, region pixels); thread l renders
+// region pixel l, frame pixel (bottom + l / width) * w + left + l % width.
 __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, uint32_t w,
-                                                   uint32_t h, uint32_t spp, uint32_t sample,
-                                                   uint32_t seed, float4* __restrict__ pixels,
+                                                   uint32_t h, uint4 region, uint32_t spp,
+                                                   uint32_t sample, uint32_t seed,
+                                                   float4* __restrict__ pixels,
                                                    unsigned long long* __restrict__ nrays_total) {
   extern __shared__ float2 stk_mem[];
   float2* stk = stk_mem + threadIdx.x;
   const uint32_t stride = blockDim.x;
-  const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nrays = 0;
-  if (pix < w * h) {
-    const uint32_t x = pix % w, y = pix / w;
+  if (l < region.w) {
+    const uint32_t x = region.x + l % region.z, y = region.y + l / region.z;
+    const uint32_t pix = y * w + x;
     CompatRng rng{seed, pix, sample, 0u, U4{0, 0, 0, 0}};
     const int kMaxDepth = 8;
     CRay stack[kMaxDepth + 2];
@@ -353,6 +357,11 @@ __device__ __forceinline__ const JobDev& job_of(const PbrArgs& a, uint32_t q) {
     return a.job;
 }
 
+// Frame pixel index of region pixel l.
+__device__ __forceinline__ uint32_t frame_pixel(const JobDev& J, uint32_t l) {
+  return (J.ry0 + l / J.rw) * J.w + J.rx0 + l % J.rw;
+}
+
 __device__ __forceinline__ uint32_t job_shader(const JobDev& J, uint32_t tri) {
   uint32_t sid = J.shader_ids[tri];
   sid = sid > 0 ? sid - 1 : 0;  // getSafeID (src/util/array.h:18)
@@ -383,8 +392,9 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   }
   const JobDev& J = job_at<kMulti>(a, jb);
   const uint32_t r = (uint32_t)(g - J.path_start);
-  const uint32_t p = r % J.npix, smp = r / J.npix;
-  const uint32_t x = p % J.w, y = p / J.w;
+  const uint32_t l = r % J.npix, smp = r / J.npix;
+  const uint32_t x = J.rx0 + l % J.rw, y = J.ry0 + l / J.rw;
+  const uint32_t p = y * J.w + x;  // frame pixel: the random-number key
   const U4 rn = pbr_draw(J.seed, p, smp, 0, 0);
   const float j0 = u01(rn.x), j1 = u01(rn.y);
   uint32_t sw = 1;
@@ -447,7 +457,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   const uint32_t p = qin[j];  // path index in the pass
   const JobDev& J = job_of<kMulti>(a, p);
   const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
-  const uint32_t pix = r0 % J.npix, smp = r0 / J.npix;
+  const uint32_t pix = frame_pixel(J, r0 % J.npix), smp = r0 / J.npix;
   const size_t base = j, ss = a.cap;  // slot e of queue entry j at base + e * ss
   const size_t np = a.cap;
   const aur_hit hit = a.hits[p];
@@ -636,7 +646,7 @@ __global__ void k_pbr_accumulate(PbrArgs a) {
   const JobDev& J = job_of<kMulti>(a, q);
   const uint32_t r = (uint32_t)(g0 + q - J.path_start);
   if (q >= J.npix && r >= J.npix) return;  // the pixel's previous sample is in this pass
-  const uint32_t pix = r % J.npix;
+  const uint32_t pix = frame_pixel(J, r % J.npix);
   const float weight = 1.0f / (float)J.spp;
   float4 px = J.pixels[pix];
   for (uint32_t qq = q, rr = r; qq < n && rr < J.paths; qq += J.npix, rr += J.npix) {
@@ -724,7 +734,7 @@ __global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, const float4* __re
   const uint32_t lbase = 3u * J.num_shaders;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; scale > 0.0f && q < np;
        q += gridDim.x * blockDim.x) {
-    const uint32_t pix = (uint32_t)(g0 + q - J.path_start) % J.npix;
+    const uint32_t pix = frame_pixel(J, (uint32_t)(g0 + q - J.path_start) % J.npix);
     const float4 g = dimg[pix];
     const V3 dl = mk(g.x * inv_spp, g.y * inv_spp, g.z * inv_spp);
     if (dl.x == 0.0f && dl.y == 0.0f && dl.z == 0.0f) continue;
@@ -1029,7 +1039,10 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   J.nslots = ctx->num_slots;
   J.w = fr.width;
   J.h = fr.height;
-  J.npix = fr.width * fr.height;
+  J.rx0 = fr.left;
+  J.ry0 = fr.bottom;
+  J.rw = fr.right - fr.left + 1;
+  J.npix = J.rw * (fr.top - fr.bottom + 1);
   J.spp = o->spp;
   J.max_depth = o->max_depth;
   J.rr_start = o->rr_start;
@@ -1039,7 +1052,51 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   return J;
 }
 
+// 3x3 tent filter of the region (cudaFilterPixelsKernel,
+// src/kernels/utility.cu:231-258): weights 1/16, 1/8, 1/4, taps clamped to
+// the region, sums in the reference's order.  The reference clamps y + 1 to
+// region.bottom (F9), which this build fixes to region.top.
+__global__ void k_filter_pixels(uint32_t w, int left, int right, int bottom, int top,
+                                const float4* __restrict__ in, float4* __restrict__ out) {
+  const int x = left + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = bottom + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (x > right || y > top) return;
+  const int xm = max(x - 1, left), xp = min(x + 1, right);
+  const int ym = max(y - 1, bottom), yp = min(y + 1, top);
+  auto at = [&](int yy, int xx) { return in[(size_t)yy * w + xx]; };
+  const float4 t[9] = {at(ym, xm), at(ym, x), at(ym, xp), at(y, xm), at(y, x),
+                       at(y, xp),  at(yp, xm), at(yp, x), at(yp, xp)};
+  const float wt[9] = {0.0625f, 0.125f, 0.0625f, 0.125f, 0.25f, 0.125f, 0.0625f, 0.125f, 0.0625f};
+  float4 v = make_float4(t[0].x * wt[0], t[0].y * wt[0], t[0].z * wt[0], t[0].w * wt[0]);
+  for (int i = 1; i < 9; ++i) {
+    v.x = v.x + t[i].x * wt[i];
+    v.y = v.y + t[i].y * wt[i];
+    v.z = v.z + t[i].z * wt[i];
+    v.w = v.w + t[i].w * wt[i];
+  }
+  out[(size_t)y * w + x] = v;
+}
+
 }  // namespace
+
+aur_status frame_filter(aur_context* ctx) {
+  FrameState& fr = ctx->frame;
+  if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "filter: no frame");
+  const size_t n = (size_t)fr.width * fr.height;
+  DevBuf<float4> out;
+  AUR_HIP(ctx, out.reserve(n));
+  // pixels outside the region keep their values
+  AUR_HIP(ctx, hipMemcpyAsync(out.p, fr.pixels.p, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+  const dim3 block(32, 8);
+  const dim3 grid(nblocks(fr.right - fr.left + 1, 32), nblocks(fr.top - fr.bottom + 1, 8));
+  k_filter_pixels<<<grid, block, 0, ctx->stream>>>(fr.width, (int)fr.left, (int)fr.right, (int)fr.bottom,
+                                                   (int)fr.top, fr.pixels.p, out.p);
+  AUR_LAUNCH_CHECK(ctx, "k_filter_pixels");
+  // back into the accumulation buffer, so that its device pointer stays valid
+  AUR_HIP(ctx, hipMemcpyAsync(fr.pixels.p, out.p, n * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+  AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return AUR_OK;
+}
 
 aur_status pbr_run(aur_context* ctx, const PbrRun& run, unsigned long long* stats) {
   if (run.njobs == 0 || run.end <= run.begin) return AUR_OK;
@@ -1057,7 +1114,8 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   if (o->mode == AUR_MODE_PBR && o->max_depth == 0) return ctx->fail(AUR_ERR_INVALID_ARG, "render: max_depth 0");
   if (o->mode != AUR_MODE_COMPAT && o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "render: mode");
   hipStream_t s = ctx->stream;
-  const uint32_t w = fr.width, h = fr.height, npix = w * h;
+  const uint32_t w = fr.width, h = fr.height;
+  const uint32_t rw = fr.right - fr.left + 1, rh = fr.top - fr.bottom + 1, npix = rw * rh;
   if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
   AUR_HIP(ctx, fr.counters.reserve(8));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
@@ -1066,8 +1124,9 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
     SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
                 ctx->num_shaders, ctx->lights.p, ctx->num_lights};
     const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
+    const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
     for (uint32_t smp = s0; smp < s1; ++smp) {
-      k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, o->spp, smp, o->seed,
+      k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, region, o->spp, smp, o->seed,
                                                           fr.pixels.p, stats);
     }
     AUR_LAUNCH_CHECK(ctx, "k_compat");
@@ -1110,7 +1169,8 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   const uint32_t nparams = 3u * (ctx->num_shaders + ctx->num_lights);
   if (nparams > kAdjMaxParams) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: more than 2048 shaders + lights");
   hipStream_t s = ctx->stream;
-  const uint32_t npix = fr.width * fr.height;
+  const uint32_t npix = (fr.right - fr.left + 1) * (fr.top - fr.bottom + 1);  // region pixels
+  const uint32_t nframe = fr.width * fr.height;
   if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: pixels * spp exceeds 2^32");
   AUR_HIP(ctx, fr.counters.reserve(8));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
@@ -1124,7 +1184,7 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   adj.absmax = fr.adj_absmax.p;
   k_zero_f64<<<nblocks(nparams, 256), 256, 0, s>>>(adj.total, nparams);
   k_adj_zero_u32<<<1, 64, 0, s>>>(adj.absmax);
-  k_adj_absmax<<<std::min<uint32_t>(nblocks(npix, 256), 1024), 256, 0, s>>>(adj.dimg, npix, adj.absmax);
+  k_adj_absmax<<<std::min<uint32_t>(nblocks(nframe, 256), 1024), 256, 0, s>>>(adj.dimg, nframe, adj.absmax);
   AUR_LAUNCH_CHECK(ctx, "adjoint setup");
   const JobDev J = frame_job(ctx, cam, o);
   PbrRun run;

@@ -67,6 +67,9 @@ JOB_DTYPE = np.dtype({
     "itemsize": 104,
 })
 
+# aur_transform (include/aurora_amd.h): the reference's Transform, 136 B
+TRANSFORM_DTYPE = scenes.TRANSFORM_DTYPE
+
 _lib = None
 
 
@@ -118,6 +121,12 @@ def load():
         "aur_frame_rays_traced": ([P], u64),
         "aur_frame_ray_counts": ([P, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
         "aur_frame_copy": ([P, P], i32),
+        "aur_frame_set_region": ([P, u32, u32, u32, u32], i32),
+        "aur_frame_get_region": ([P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                  ctypes.POINTER(u32)], i32),
+        "aur_frame_filter": ([P], i32),
+        "aur_frame_write": ([P, P], i32),
+        "aur_geometry_upload_objects": ([P, u32, P, P, P, P, u32, P], i32),
         "aur_scene_add": ([P, u32, P, P, P, u32, P, u32, P, ctypes.POINTER(u32)], i32),
         "aur_scene_clear": ([P], i32),
         "aur_scene_count": ([P], u32),
@@ -194,6 +203,20 @@ class Context:
         self._check(self._lib.aur_geometry_upload(self._h, n, _ptr(v), _ptr(nrm), _ptr(uv), _ptr(ids)),
                     "geometry upload")
         self.count = n
+
+    def upload_objects(self, vertices, normals, transforms, texcoords=None, shader_ids=None) -> None:
+        """Object-space triangles + per-object transforms (TRANSFORM_DTYPE);
+        an odd count is padded to even on upload."""
+        v = np.ascontiguousarray(vertices, np.float32)
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        uv = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32)
+        ids = None if shader_ids is None else np.ascontiguousarray(shader_ids, np.uint16)
+        xf = np.ascontiguousarray(transforms, dtype=TRANSFORM_DTYPE)
+        self._check(self._lib.aur_geometry_upload_objects(self._h, v.shape[0], _ptr(v), _ptr(n), _ptr(uv),
+                                                          _ptr(ids), xf.shape[0],
+                                                          _ptr(xf) if xf.shape[0] else None),
+                    "upload_objects")
+        self.count = v.shape[0] + (v.shape[0] & 1)
 
     def upload_shaders(self, shaders: np.ndarray) -> None:
         s = np.ascontiguousarray(shaders, dtype=scenes.SHADER_DTYPE)
@@ -318,6 +341,23 @@ class Context:
         out = np.empty((self.height * self.width, 4), dtype=np.float32)
         self._check(self._lib.aur_frame_read(self._h, _ptr(out)), "frame_read")
         return out
+
+    def frame_set_region(self, left: int, right: int, bottom: int, top: int) -> None:
+        self._check(self._lib.aur_frame_set_region(self._h, left, right, bottom, top), "frame_set_region")
+
+    def frame_get_region(self) -> tuple[int, int, int, int]:
+        v = [ctypes.c_uint32() for _ in range(4)]
+        self._check(self._lib.aur_frame_get_region(self._h, *(ctypes.byref(x) for x in v)), "frame_get_region")
+        return tuple(int(x.value) for x in v)
+
+    def frame_filter(self) -> None:
+        self._check(self._lib.aur_frame_filter(self._h), "frame_filter")
+
+    def frame_write(self, rgba: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        if a.size != self.width * self.height * 4:
+            raise ValueError("frame_write: need width*height*4 floats")
+        self._check(self._lib.aur_frame_write(self._h, _ptr(a)), "frame_write")
 
     def frame_device_buffer(self) -> int:
         p = ctypes.c_void_p()

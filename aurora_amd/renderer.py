@@ -130,14 +130,65 @@ class MonteCarloRaytracer(Renderer):
         return Status.kSuccess
 
     def setRegion(self, region: Rect) -> Status:  # noqa: N802
-        # the reference stores the region but its ray generation always maps
-        # the full frame (utility.cu:79-94); only full-frame regions exist here
         if self.m_size is None:
             return Status.kFailure
-        if (region.left, region.bottom) != (0, 0) or region.right != self.m_size.width - 1 or \
-                region.top != self.m_size.height - 1:
-            return Status.kFailure
+        try:
+            self.ctx.frame_set_region(region.left, region.right, region.bottom, region.top)
+        except native.AuroraError as e:
+            return _status_of(e)
         self.m_region = region
+        return Status.kSuccess
+
+    def filterPixels(self) -> Status:  # noqa: N802
+        """Renderer::filterPixels (src/core/renderer.cpp:43-46): 3x3 tent over
+        the region, in place."""
+        try:
+            self.ctx.frame_filter()
+        except native.AuroraError as e:
+            return _status_of(e)
+        return Status.kSuccess
+
+    # -- progressive (IPR) rendering ------------------------------------------
+    # The reference's IPR thread calls render(ipr=True) in a loop, restarting
+    # from zero samples every time (src/core/engine.cpp:282-305).  Here the
+    # accumulation persists: each step adds the next samples of a fixed
+    # budget, and a checkpoint (accumulation + samples done) resumes to the
+    # same image an uninterrupted render gives.
+
+    def beginProgressive(self, total_samples: int) -> Status:  # noqa: N802
+        if self.m_size is None:
+            return Status.kFailure
+        self.m_total = int(total_samples)
+        self.m_done = 0
+        try:
+            self.ctx.frame_clear()
+        except native.AuroraError as e:
+            return _status_of(e)
+        return Status.kSuccess
+
+    def progressiveStep(self, samples: int = 1) -> Status:  # noqa: N802
+        """Adds the next `samples` samples; the image is the running estimate
+        weighted for the full budget (scale by total / done to view it)."""
+        s0, s1 = self.m_done, min(self.m_done + int(samples), self.m_total)
+        if s1 <= s0:
+            return Status.kSuccess
+        opts = native.Context.opts(self.mode, self.m_total, self.max_depth, self.rr_start, self.seed)
+        try:
+            self.ctx.frame_render_samples(self.m_camera, opts, s0, s1)
+        except native.AuroraError as e:
+            return _status_of(e)
+        self.m_done = s1
+        return Status.kSuccess
+
+    def checkpoint(self) -> dict:
+        return {"samples_done": self.m_done, "total": self.m_total, "rgba": self.ctx.frame_read()}
+
+    def resume(self, cp: dict) -> Status:
+        try:
+            self.ctx.frame_write(cp["rgba"])
+        except native.AuroraError as e:
+            return _status_of(e)
+        self.m_done, self.m_total = int(cp["samples_done"]), int(cp["total"])
         return Status.kSuccess
 
     def render(self, engine=None, ipr: bool = False) -> Status:

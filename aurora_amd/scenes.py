@@ -230,6 +230,56 @@ def _sphere_tris(V: np.ndarray, F: np.ndarray, centre, radius, R=None):
     return P[F], U[F]
 
 
+# The reference's Transform (src/util/transform.h:14-27): first triangle and
+# triangle count of an object, row-vector world matrix (translation in row 3)
+# and its inverse transpose for normals.
+TRANSFORM_DTYPE = np.dtype([("offset", np.uint32), ("size", np.uint32), ("world", np.float32, (4, 4)),
+                            ("normal", np.float32, (4, 4))])
+assert TRANSFORM_DTYPE.itemsize == 136
+
+
+def make_transform(offset: int, size: int, world: np.ndarray) -> np.void:
+    """world: 4x4 row-vector matrix (p' = [p 1] @ world)."""
+    t = np.zeros((), dtype=TRANSFORM_DTYPE)
+    t["offset"] = offset
+    t["size"] = size
+    t["world"] = world
+    t["normal"] = np.linalg.inv(np.asarray(world, np.float64)).T
+    return t
+
+
+def instanced_spheres(seed: int = 9, n_objects: int = 12, subdivisions: int = 2, box: float = 20.0,
+                      drop_last: bool = True):
+    """Object-space unit icospheres with per-object world matrices (scale,
+    rotation, translation), as Scene::updateMeshes stages them
+    (src/core/scene.cpp:85-240).  drop_last removes the last triangle so the
+    count is odd.  Returns (vertices, normals, shader_ids, transforms)."""
+    rng = np.random.default_rng(seed)
+    V, F = icosphere(subdivisions)
+    P, Nm = _sphere_tris(V, F, (0.0, 0.0, 0.0), 1.0)
+    v1, n1 = _pack_triangles(P, Nm)
+    verts, norms, ids, xfs = [], [], [], []
+    off = 0
+    for k in range(n_objects):
+        S = np.diag(rng.uniform(0.5, 3.0, 3))  # non-uniform scale: normals need the inverse transpose
+        R = _random_rotation(rng)
+        M = np.eye(4)
+        M[:3, :3] = S @ R.T  # row vectors: p' = p S R^T
+        M[3, :3] = rng.uniform(0.0, box, 3)
+        verts.append(v1)
+        norms.append(n1)
+        ids.append(np.full(v1.shape[0], 1 + k % 4, np.uint16))
+        xfs.append(make_transform(off, v1.shape[0], M))
+        off += v1.shape[0]
+    v = np.concatenate(verts)
+    n = np.concatenate(norms)
+    i = np.concatenate(ids)
+    if drop_last:
+        v, n, i = v[:-1], n[:-1], i[:-1]
+        xfs[-1]["size"] -= 1
+    return v, n, i, np.array(xfs, dtype=TRANSFORM_DTYPE)
+
+
 def _lambert(albedo, emission=(0.0, 0.0, 0.0)) -> np.void:
     s = np.zeros((), dtype=SHADER_DTYPE)
     s["type"] = SHADER_LAMBERT

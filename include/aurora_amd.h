@@ -193,6 +193,24 @@ aur_status aur_frame_render_samples(aur_context* ctx, const aur_camera* camera,
                                     const aur_render_opts* opts, uint32_t sample_begin,
                                     uint32_t sample_end);
 aur_status aur_frame_clear(aur_context* ctx);
+/* Render region, inclusive pixel bounds (MonteCarloRaytracer::setRegion,
+ * src/render/monteCarloRaytracer.cpp:75-79; Rect left/right/bottom/top).
+ * Renders trace and accumulate only region pixels, each exactly as a full
+ * render would (the random numbers are keyed by frame pixel).
+ * aur_frame_create resets it to the whole frame. */
+aur_status aur_frame_set_region(aur_context* ctx, uint32_t left, uint32_t right, uint32_t bottom,
+                                uint32_t top);
+aur_status aur_frame_get_region(aur_context* ctx, uint32_t* left, uint32_t* right, uint32_t* bottom,
+                                uint32_t* top);
+/* 3x3 tent filter (1/16 1/8 1/16; 1/8 1/4 1/8; 1/16 1/8 1/16) of the region
+ * in place, taps clamped to the region (cudaFilterPixels,
+ * src/kernels/utility.cu:231-268; the reference clamps the row above to
+ * region.bottom, F9, and never calls it).  Synchronous. */
+aur_status aur_frame_filter(aur_context* ctx);
+/* Loads host float RGBA (width*height*4) into the accumulation buffer: with
+ * aur_frame_render_samples, resumes a progressive render from a saved
+ * accumulation and sample count; the result equals an uninterrupted render. */
+aur_status aur_frame_write(aur_context* ctx, const float* rgba);
 /* Host float RGBA, width*height*4 floats, row 0 = bottom (framebuffer(),
  * src/render/monteCarloRaytracer.cpp:122-127). */
 aur_status aur_frame_read(aur_context* ctx, float* rgba);
@@ -226,6 +244,30 @@ aur_status aur_frame_ray_counts(const aur_context* ctx, uint64_t* camera_and_ext
  * Deterministic: the same inputs give the same bits. */
 aur_status aur_frame_adjoint(aur_context* ctx, const aur_camera* camera, const aur_render_opts* opts,
                              const float* d_image, float* grad_albedo, float* grad_radiance);
+
+/* ---- object-space upload (replaces Scene::updateMeshes' staging,
+ * Geometry::padToEven and cudaTransform: src/core/scene.cpp:85-240,
+ * src/data/geometry.cpp:164-176, src/kernels/utility.cu:96-134) ----
+ * Layout of the reference's Transform (src/util/transform.h:14-27). */
+typedef struct aur_transform {
+  uint32_t offset, size; /* first triangle and triangle count of the object */
+  float world[4][4];     /* row-vector matrix, translation in row 3: p' = (p, 1) world */
+  float normal[4][4];    /* inverse transpose of world; n' = n normal[0..2][0..2] */
+} aur_transform;
+/* Uploads object-space triangles (layouts as aur_geometry_upload; count may
+ * be odd: the last triangle is duplicated to an even count, triangle-wise,
+ * where the reference copied floats, F9) and transforms them on the device:
+ * triangle t takes the last transform, in array order, whose offset <= t
+ * (the pad triangle included); positions by `world`, normals by the upper
+ * 3x3 of `normal` (not renormalised); uvs and shader ids unchanged.
+ * Triangles before every offset keep their object-space values (the
+ * reference left them unwritten).  Transformed coordinates must be finite
+ * (AUR_ERR_INVALID_ARG otherwise).  Then call aur_rebuild_nmh as after
+ * aur_geometry_upload. */
+aur_status aur_geometry_upload_objects(aur_context* ctx, uint32_t count, const float* vertices,
+                                       const float* normals, const float* texcoords,
+                                       const uint16_t* shader_ids, uint32_t num_objects,
+                                       const aur_transform* transforms);
 
 /* ---- batches (no reference counterpart: the reference renders one frame
  * of one scene per MonteCarloRaytracer::render call,

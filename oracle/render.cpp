@@ -680,6 +680,29 @@ int orc_adjoint(uint32_t count, const float* verts, const float* normals, const 
   return 0;
 }
 
+// cudaFilterPixelsKernel restated (src/kernels/utility.cu:231-258): 3x3
+// tent over the inclusive region, taps clamped to the region, float4 sums in
+// the reference's order; the row above is clamped to region.top (the
+// reference uses region.bottom, F9).  in/out: float4 per pixel, w*h; out's
+// pixels outside the region are copied from in.
+void orc_filter_pixels(uint32_t w, uint32_t h, const uint32_t* region, const float* in, float* out) {
+  std::memcpy(out, in, size_t(w) * h * 4 * sizeof(float));
+  const int left = (int)region[0], right = (int)region[1], bottom = (int)region[2], top = (int)region[3];
+  const float wt[9] = {0.0625f, 0.125f, 0.0625f, 0.125f, 0.25f, 0.125f, 0.0625f, 0.125f, 0.0625f};
+  for (int y = bottom; y <= top; ++y) {
+    for (int x = left; x <= right; ++x) {
+      const int xs[3] = {x - 1 < left ? left : x - 1, x, x + 1 > right ? right : x + 1};
+      const int ys[3] = {y - 1 < bottom ? bottom : y - 1, y, y + 1 > top ? top : y + 1};
+      float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int i = 0; i < 9; ++i) {
+        const float* p = in + (size_t(ys[i / 3]) * w + xs[i % 3]) * 4;
+        for (int c = 0; c < 4; ++c) v[c] = i == 0 ? p[c] * wt[0] : v[c] + p[c] * wt[i];
+      }
+      std::memcpy(out + (size_t(y) * w + x) * 4, v, sizeof(v));
+    }
+  }
+}
+
 // Exposed pieces for closed-form tests.
 void orc_concentric_disk(float u1, float u2, float* out) { concentric_disk(u1, u2, out[0], out[1]); }
 void orc_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {

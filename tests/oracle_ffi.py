@@ -47,6 +47,9 @@ def lib():
         L.orc_render.restype = i32
         L.orc_adjoint.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, u32, P, P, P, P, i32]
         L.orc_adjoint.restype = i32
+        L.orc_transform_objects.argtypes = [u32, P, P, u32, P, P, P]
+        L.orc_transform_objects.restype = i32
+        L.orc_filter_pixels.argtypes = [u32, u32, P, P, P]
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_philox.argtypes = [P, u32, u32, P]
         _lib = L
@@ -174,6 +177,32 @@ def adjoint(tree, scene, spp: int, max_depth: int, rr_start: int, seed: int, dim
     if rc != 0:
         raise RuntimeError(f"orc_adjoint failed: {rc}")
     return ga, gl[: li.shape[0]]
+
+
+def transform_objects(vertices: np.ndarray, normals: np.ndarray, transforms: np.ndarray):
+    """Padded, transformed (vertices, normals), float32 (N, 9) each."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    n = np.ascontiguousarray(normals, np.float32)
+    xf = np.ascontiguousarray(transforms)
+    count = v.shape[0]
+    N = count + (count & 1)
+    ov = np.zeros((N, 9), np.float32)
+    on = np.zeros((N, 9), np.float32)
+    rc = lib().orc_transform_objects(count, _p(v), _p(n), xf.shape[0],
+                                     xf.ctypes.data_as(ctypes.c_void_p) if xf.shape[0] else None,
+                                     _p(ov), _p(on))
+    if rc != 0:
+        raise RuntimeError(f"orc_transform_objects failed: {rc}")
+    return ov, on
+
+
+def filter_pixels(img: np.ndarray, w: int, h: int, region) -> np.ndarray:
+    """img float32 (w*h, 4); region (left, right, bottom, top) inclusive."""
+    a = np.ascontiguousarray(img, np.float32).reshape(w * h, 4)
+    out = np.empty_like(a)
+    r = np.ascontiguousarray(region, np.uint32)
+    lib().orc_filter_pixels(w, h, _p(r), _p(a), _p(out))
+    return out
 
 
 def tree_arrays(scene, perm: np.ndarray) -> dict:
