@@ -28,6 +28,9 @@ aur_status aur_scene_add(aur_context* ctx, uint32_t count, const float* vertices
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (!scene_id) return ctx->fail(AUR_ERR_INVALID_ARG, "scene: null id");
   if (ctx->scenes.size() >= 65536) return ctx->fail(AUR_ERR_INVALID_ARG, "scene: more than 65536 scenes");
+  for (uint32_t i = 0; i < num_shaders && shaders; ++i)
+    if (shaders[i].texture != 0)
+      return ctx->fail(AUR_ERR_UNSUPPORTED, "scene: texture-mapped shaders are not kept for batches");
   // geometry, shaders and lights are checked by the single-scene entry points
   aur_status st = aur_geometry_upload(ctx, count, vertices, normals, nullptr, shader_ids);
   if (st != AUR_OK) return st;

@@ -146,16 +146,44 @@ aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* ve
 aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader* shaders) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (count == 0 || !shaders) return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: empty");
+  bool textured = false;
   for (uint32_t i = 0; i < count; ++i) {
-    if (shaders[i].texture != 0)
-      return ctx->fail(AUR_ERR_UNSUPPORTED, "shaders: texture maps are not supported");
     if (shaders[i].type != 0 && shaders[i].type != 1)
       return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: unknown type");
+    textured = textured || shaders[i].texture != 0;
   }
   AUR_HIP(ctx, hipSetDevice(ctx->device));
   AUR_HIP(ctx, ctx->shaders.reserve(count));
   AUR_HIP(ctx, hipMemcpy(ctx->shaders.p, shaders, count * sizeof(aur_shader), hipMemcpyHostToDevice));
   ctx->num_shaders = count;
+  ctx->textured_shaders = textured;
+  return AUR_OK;
+}
+
+aur_status aur_textures_upload(aur_context* ctx, uint32_t count, const aur_texture* textures,
+                               const uint8_t* texels, uint64_t texel_bytes) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (count && (!textures || !texels)) return ctx->fail(AUR_ERR_INVALID_ARG, "textures: null");
+  std::vector<aur::TexDesc> desc(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const aur_texture& t = textures[i];
+    if (t.width == 0 || t.height == 0 || t.width > 65536 || t.height > 65536)
+      return ctx->fail(AUR_ERR_INVALID_ARG, "textures: size must be 1..65536 per side");
+    const uint64_t bytes = 4ull * t.width * t.height;
+    if (t.offset > texel_bytes || bytes > texel_bytes - t.offset)
+      return ctx->fail(AUR_ERR_INVALID_ARG, "textures: texels outside the buffer");
+    desc[i] = aur::TexDesc{t.offset, t.width, t.height};
+  }
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->num_textures = 0;
+  if (count) {
+    AUR_HIP(ctx, ctx->textures.reserve(count));
+    AUR_HIP(ctx, ctx->texels.reserve(texel_bytes));
+    AUR_HIP(ctx, hipMemcpy(ctx->textures.p, desc.data(), count * sizeof(aur::TexDesc), hipMemcpyHostToDevice));
+    AUR_HIP(ctx, hipMemcpy(ctx->texels.p, texels, texel_bytes, hipMemcpyHostToDevice));
+  }
+  ctx->num_textures = count;
   return AUR_OK;
 }
 

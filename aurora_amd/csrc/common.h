@@ -44,6 +44,44 @@ __host__ __device__ inline V3 bclerp(V3 a, V3 b, V3 c, float u, float v) {
   return (1.0f - u - v) * a + u * b + v * c;
 }
 __host__ __device__ inline V3 reflect(V3 i, V3 n) { return i - 2.0f * n * dot(n, i); }
+
+// RGBA8 texture: texels at byte offset `offset` of the texel buffer,
+// row-major, row y at offset + 4 * width * y.
+struct TexDesc {
+  unsigned long long offset;
+  uint32_t width, height;
+};
+
+// Bilinear fetch with wrap addressing on normalised coordinates, texels
+// read as value / 255 (the filtering the reference configures for tex2D,
+// src/kernels/lib/texunit.cuh:12-19): x = frac(u) * width - 0.5, neighbours
+// wrap, weights are float products.  (The texture unit quantises the weights
+// to 1/256; that rounding is not reproduced.)
+__host__ __device__ inline V3 tex_sample(const TexDesc& t, const uint8_t* texels, float u, float v) {
+  float uf = u - floorf(u), vf = v - floorf(v);
+  if (!(uf >= 0.0f && uf <= 1.0f)) uf = 0.0f;  // non-finite coordinates
+  if (!(vf >= 0.0f && vf <= 1.0f)) vf = 0.0f;
+  const float x = uf * (float)t.width - 0.5f, y = vf * (float)t.height - 0.5f;
+  const float x0f = floorf(x), y0f = floorf(y);
+  const float ax = x - x0f, ay = y - y0f;
+  const int W = (int)t.width, H = (int)t.height;
+  int x0 = (int)x0f % W, y0 = (int)y0f % H;
+  if (x0 < 0) x0 += W;
+  if (y0 < 0) y0 += H;
+  const int x1 = x0 + 1 == W ? 0 : x0 + 1, y1 = y0 + 1 == H ? 0 : y0 + 1;
+  const uint8_t* base = texels + t.offset;
+  const uint8_t* p00 = base + 4 * ((size_t)y0 * W + x0);
+  const uint8_t* p10 = base + 4 * ((size_t)y0 * W + x1);
+  const uint8_t* p01 = base + 4 * ((size_t)y1 * W + x0);
+  const uint8_t* p11 = base + 4 * ((size_t)y1 * W + x1);
+  const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay), w01 = (1.0f - ax) * ay, w11 = ax * ay;
+  const float k = 1.0f / 255.0f;
+  V3 c;
+  c.x = (((w00 * (float)p00[0] + w10 * (float)p10[0]) + w01 * (float)p01[0]) + w11 * (float)p11[0]) * k;
+  c.y = (((w00 * (float)p00[1] + w10 * (float)p10[1]) + w01 * (float)p01[1]) + w11 * (float)p11[1]) * k;
+  c.z = (((w00 * (float)p00[2] + w10 * (float)p10[2]) + w01 * (float)p01[2]) + w11 * (float)p11[2]) * k;
+  return c;
+}
 __host__ __device__ inline bool refract(V3& r, V3 i, V3 n, float eta1, float eta2) {
   const float eta = eta1 / eta2;
   const float cosi = dot(n, i);

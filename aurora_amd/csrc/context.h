@@ -66,6 +66,10 @@ struct JobDev {
   const aur_shader* shaders;
   const aur_light* lights;
   float4* pixels;  // accumulation, npix float4
+  const float* texcoords;  // tree order, 6 floats per triangle
+  const TexDesc* textures;
+  const uint8_t* texels;
+  uint32_t num_textures;
   unsigned long long path_start;
   uint32_t n_nodes, num_shaders, num_lights, nslots;
   uint32_t w, h, npix, spp, max_depth, rr_start, seed, paths;
@@ -144,6 +148,11 @@ struct aur_context {
   aur::DevBuf<aur_light> lights;
   uint32_t num_lights = 0;
   uint32_t num_slots = 0;  // next-event slots per path vertex
+  bool textured_shaders = false;  // a shader uses a texture map
+  // RGBA8 textures (aur_textures_upload)
+  aur::DevBuf<aur::TexDesc> textures;
+  aur::DevBuf<uint8_t> texels;
+  uint32_t num_textures = 0;
 
   aur::FrameState frame;
 

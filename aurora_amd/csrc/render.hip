@@ -40,7 +40,26 @@ struct SceneDev {
   uint32_t num_shaders;
   const aur_light* lights;
   uint32_t num_lights;
+  const float* texcoords;
+  const TexDesc* textures;
+  const uint8_t* texels;
+  uint32_t num_textures;
 };
+
+// Texture colour of a hit: texture sh.texture (1-based) at the barycentric
+// interpolation of the triangle's uvs (getTexCoord, src/kernels/lib/common.cuh:45-51;
+// texfetch, src/kernels/lib/texunit.cuh:54-94); white when no such texture
+// exists (texfetch's default).
+__device__ inline V3 tex_color(const float* texcoords, const TexDesc* textures, const uint8_t* texels,
+                               uint32_t num_textures, uint32_t texture, uint32_t tri, float u, float v) {
+  const uint32_t id = texture - 1;
+  if (id >= num_textures) return mk(1.0f, 1.0f, 1.0f);
+  const float* t = texcoords + (size_t)tri * 6;
+  const float w = 1.0f - u - v;
+  const float tu = (w * t[0] + u * t[2]) + v * t[4];
+  const float tv = (w * t[1] + u * t[3]) + v * t[5];
+  return tex_sample(textures[id], texels, tu, tv);
+}
 
 __device__ inline uint32_t shader_index(const SceneDev& sc, uint32_t tri) {
   uint32_t sid = sc.shader_ids[tri];
@@ -143,7 +162,10 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
       const V3 wo = -ray.dir;
       const aur_shader& sh = shader_of(sc, (uint32_t)hpId);
       const V3 N = normalize(interp9(sc.normals, hpId, hpU, hpV));
-      const V3 color1 = ld3(sh.diffuse_color) * sh.diffuse;
+      V3 color1 = ld3(sh.diffuse_color) * sh.diffuse;
+      if (sh.texture > 0)
+        color1 = tex_color(sc.texcoords, sc.textures, sc.texels, sc.num_textures, sh.texture, (uint32_t)hpId,
+                           hpU, hpV) * sh.diffuse;
       V3 Li = ld3(sh.emission_color);
       for (uint32_t i = 0; i < sc.num_lights; ++i) {
         const aur_light& L = sc.lights[i];
@@ -477,7 +499,10 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   const aur_shader& sh = J.shaders[sid];
   V3 N = normalize(interp9(J.normals, (uint32_t)hit.id, hit.u, hit.v));
   if (dot(N, wo) < 0.0f) N = -N;
-  const V3 albedo = ld3(sh.diffuse_color) * sh.diffuse;
+  V3 albedo = ld3(sh.diffuse_color) * sh.diffuse;
+  if (sh.texture > 0)
+    albedo = tex_color(J.texcoords, J.textures, J.texels, J.num_textures, sh.texture, (uint32_t)hit.id, hit.u,
+                       hit.v) * sh.diffuse;
   const V3 f = albedo * kInvPi;
   V3 L = mk(L4.x, L4.y, L4.z) + beta * ld3(sh.emission_color);
   a.radiance[p] = make_float4(L.x, L.y, L.z, bounce == 0 ? 1.0f : L4.w);
@@ -1032,6 +1057,10 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   J.shaders = ctx->shaders.p;
   J.lights = ctx->lights.p;
   J.pixels = fr.pixels.p;
+  J.texcoords = ctx->texcoords.p;
+  J.textures = ctx->textures.p;
+  J.texels = ctx->texels.p;
+  J.num_textures = ctx->num_textures;
   J.path_start = 0;
   J.n_nodes = ctx->count / 2;
   J.num_shaders = ctx->num_shaders;
@@ -1122,7 +1151,8 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   if (o->mode == AUR_MODE_COMPAT) {
     SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
-                ctx->num_shaders, ctx->lights.p, ctx->num_lights};
+                ctx->num_shaders, ctx->lights.p, ctx->num_lights, ctx->texcoords.p, ctx->textures.p,
+                ctx->texels.p, ctx->num_textures};
     const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
     const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
     for (uint32_t smp = s0; smp < s1; ++smp) {
@@ -1168,6 +1198,8 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
     return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: null buffer");
   const uint32_t nparams = 3u * (ctx->num_shaders + ctx->num_lights);
   if (nparams > kAdjMaxParams) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: more than 2048 shaders + lights");
+  if (ctx->textured_shaders)
+    return ctx->fail(AUR_ERR_UNSUPPORTED, "adjoint: texture-mapped shaders (albedo is not a parameter there)");
   hipStream_t s = ctx->stream;
   const uint32_t npix = (fr.right - fr.left + 1) * (fr.top - fr.bottom + 1);  // region pixels
   const uint32_t nframe = fr.width * fr.height;

@@ -99,6 +99,7 @@ def load():
         "aur_geometry_upload": ([P, u32, P, P, P, P], i32),
         "aur_shaders_upload": ([P, u32, P], i32),
         "aur_lights_upload": ([P, u32, P], i32),
+        "aur_textures_upload": ([P, u32, P, P, u64], i32),
         "aur_rebuild_nmh": ([P, ctypes.POINTER(BuildStats)], i32),
         "aur_generate_tb": ([P], i32),
         "aur_read_permutation": ([P, P], i32),
@@ -217,6 +218,13 @@ class Context:
                                                           _ptr(xf) if xf.shape[0] else None),
                     "upload_objects")
         self.count = v.shape[0] + (v.shape[0] & 1)
+
+    def upload_textures(self, textures: np.ndarray, texels: np.ndarray) -> None:
+        """textures: scenes.TEXTURE_DTYPE records; texels: uint8 RGBA bytes."""
+        t = np.ascontiguousarray(textures, dtype=scenes.TEXTURE_DTYPE)
+        b = np.ascontiguousarray(texels, dtype=np.uint8)
+        self._check(self._lib.aur_textures_upload(self._h, t.shape[0], _ptr(t) if t.shape[0] else None,
+                                                  _ptr(b) if b.size else None, b.size), "upload_textures")
 
     def upload_shaders(self, shaders: np.ndarray) -> None:
         s = np.ascontiguousarray(shaders, dtype=scenes.SHADER_DTYPE)

@@ -365,6 +365,38 @@ def _finish(parts, ids, shaders, lights, camera, **kw) -> Scene:
     )
 
 
+# RGBA8 texture descriptor (include/aurora_amd.h aur_texture)
+TEXTURE_DTYPE = np.dtype([("width", np.uint32), ("height", np.uint32), ("offset", np.uint64)])
+assert TEXTURE_DTYPE.itemsize == 16
+
+
+def textured(scene: Scene, seed: int = 4):
+    """Adds texture maps to a scene (as Scene::updateTextures feeds them,
+    src/core/scene.cpp:298-326): per-vertex uvs from a planar projection of
+    the positions (values outside [0,1] exercise wrap addressing), a
+    checkerboard 8x8, a 5x3 noise texture and, for one shader, a texture
+    index past the uploaded count (sampled as white).
+    Returns (scene, textures (TEXTURE_DTYPE), texels uint8)."""
+    rng = np.random.default_rng(seed)
+    s = dataclasses.replace(scene)
+    P = np.transpose(s.vertices.reshape(-1, 3, 3), (0, 2, 1))  # (T, vertex, xyz)
+    uv = np.stack([0.37 * P[..., 0] + 0.11 * P[..., 1], 0.29 * P[..., 2] - 0.23 * P[..., 1]], axis=2)
+    s.texcoords = uv.reshape(-1, 6).astype(np.float32)
+    check = np.zeros((8, 8, 4), np.uint8)
+    yy, xx = np.mgrid[0:8, 0:8]
+    on = (xx + yy) % 2 == 0
+    check[on] = (230, 200, 40, 255)
+    check[~on] = (30, 60, 210, 255)
+    noise = rng.integers(0, 256, (3, 5, 4)).astype(np.uint8)
+    texels = np.concatenate([check.reshape(-1), noise.reshape(-1)])
+    tex = np.array([(8, 8, 0), (5, 3, check.size)], dtype=TEXTURE_DTYPE)
+    sh = s.shaders.copy()
+    for i in range(sh.shape[0]):
+        sh[i]["texture"] = (1, 2, 0, 7)[i % 4]  # 7: no such texture -> white
+    s.shaders = sh
+    return s, tex, texels
+
+
 def canonical(a: np.ndarray) -> np.ndarray:
     """-0.0 -> +0.0 (float key order then equals the radix order)."""
     return (a + np.float32(0.0)).astype(np.float32)

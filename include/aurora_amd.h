@@ -121,6 +121,19 @@ aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* ve
                                const uint16_t* shader_ids);
 aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader* shaders);
 aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* lights);
+/* RGBA8 textures (replaces Scene::updateTextures / Texture::load,
+ * src/core/scene.cpp:298-326, src/data/texture.cpp:15-30; sampled as
+ * texfetch, src/kernels/lib/texunit.cuh:54-94).  Texture i (shader.texture =
+ * i + 1) is width x height texels of 4 bytes, row-major, at byte `offset` of
+ * `texels`.  Bilinear with wrap addressing on normalised coordinates, value
+ * / 255; a shader whose texture index exceeds the count samples white.
+ * Replaces all textures of the context. */
+typedef struct aur_texture {
+  uint32_t width, height;
+  uint64_t offset;
+} aur_texture;
+aur_status aur_textures_upload(aur_context* ctx, uint32_t count, const aur_texture* textures,
+                               const uint8_t* texels, uint64_t texel_bytes);
 
 /* ---- NMH build (replaces cudaRebuildNMH(Geometry&), src/kernels/rebuilder.cu:311-450,
  * declared at src/kernels/kernels.h:313).  Reorders the uploaded geometry into

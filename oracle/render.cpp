@@ -64,6 +64,12 @@ static_assert(sizeof(Light) == 80, "light layout");
 
 inline f3 v3(const float* p) { return mk3(p[0], p[1], p[2]); }
 
+struct TexDesc {  // layout of the C ABI's aur_texture: RGBA8 texels at byte offset, row-major
+  uint32_t width, height;
+  uint64_t offset;
+};
+static_assert(sizeof(TexDesc) == 16, "texture descriptor layout");
+
 struct Scene {
   uint32_t count;
   const float *verts, *normals, *tangents, *bitangents;
@@ -72,7 +78,53 @@ struct Scene {
   const Shader* shaders;
   uint32_t nLights;
   const Light* lights;
+  const float* texcoords = nullptr;  // tree order, 6 per triangle
+  uint32_t nTextures = 0;
+  const TexDesc* textures = nullptr;
+  const uint8_t* texels = nullptr;
 };
+
+// texfetch (src/kernels/lib/texunit.cuh:54-94) with the filtering the
+// reference configures (linear, wrap, normalised coordinates, :12-19) as an
+// own bilinear: x = frac(u) * width - 0.5, wrapped neighbours, float weights
+// (the texture unit's 1/256 weight quantisation is not reproduced).
+f3 tex_sample(const TexDesc& t, const uint8_t* texels, float u, float v) {
+  float uf = u - floorf(u), vf = v - floorf(v);
+  if (!(uf >= 0.0f && uf <= 1.0f)) uf = 0.0f;
+  if (!(vf >= 0.0f && vf <= 1.0f)) vf = 0.0f;
+  const float x = uf * (float)t.width - 0.5f, y = vf * (float)t.height - 0.5f;
+  const float x0f = floorf(x), y0f = floorf(y);
+  const float ax = x - x0f, ay = y - y0f;
+  const int W = (int)t.width, H = (int)t.height;
+  int x0 = (int)x0f % W, y0 = (int)y0f % H;
+  if (x0 < 0) x0 += W;
+  if (y0 < 0) y0 += H;
+  const int x1 = x0 + 1 == W ? 0 : x0 + 1, y1 = y0 + 1 == H ? 0 : y0 + 1;
+  const uint8_t* base = texels + t.offset;
+  const uint8_t* p00 = base + 4 * (size_t(y0) * W + x0);
+  const uint8_t* p10 = base + 4 * (size_t(y0) * W + x1);
+  const uint8_t* p01 = base + 4 * (size_t(y1) * W + x0);
+  const uint8_t* p11 = base + 4 * (size_t(y1) * W + x1);
+  const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay), w01 = (1.0f - ax) * ay, w11 = ax * ay;
+  const float k = 1.0f / 255.0f;
+  f3 c;
+  c.x = (((w00 * (float)p00[0] + w10 * (float)p10[0]) + w01 * (float)p01[0]) + w11 * (float)p11[0]) * k;
+  c.y = (((w00 * (float)p00[1] + w10 * (float)p10[1]) + w01 * (float)p01[1]) + w11 * (float)p11[1]) * k;
+  c.z = (((w00 * (float)p00[2] + w10 * (float)p10[2]) + w01 * (float)p01[2]) + w11 * (float)p11[2]) * k;
+  return c;
+}
+
+// Texture colour of a hit (getTexCoord, src/kernels/lib/common.cuh:45-51,
+// bclerp's order); white when the texture index exceeds the uploaded count.
+f3 tex_color(const Scene& sc, uint32_t texture, uint32_t tri, float u, float v) {
+  const uint32_t id = texture - 1;
+  if (id >= sc.nTextures || !sc.texcoords) return mk3(1.0f);
+  const float* t = sc.texcoords + size_t(tri) * 6;
+  const float w = 1.0f - u - v;
+  const float tu = (w * t[0] + u * t[2]) + v * t[4];
+  const float tv = (w * t[1] + u * t[3]) + v * t[5];
+  return tex_sample(sc.textures[id], sc.texels, tu, tv);
+}
 
 // ---- traversal (same restatement as trace.cpp, kept local so that the
 // path loop below reads like the reference) ----
@@ -290,7 +342,8 @@ Sample shade_compat(const Scene& sc, f3 origin, f3 dir, CompatRng& rng) {
     const f3 wo = -ray.dir;
     const Shader& sh = shader_of(sc, (uint32_t)hpId);
     const f3 N = normalize(interp(sc.normals, hpId, hpU, hpV));
-    const f3 color1 = v3(sh.diffuseColor) * sh.diffuse;
+    f3 color1 = v3(sh.diffuseColor) * sh.diffuse;
+    if (sh.texture > 0) color1 = tex_color(sc, sh.texture, (uint32_t)hpId, hpU, hpV) * sh.diffuse;
     f3 Li = v3(sh.emissionColor);
     for (uint32_t i = 0; i < sc.nLights; ++i) {
       const Light& L = sc.lights[i];
@@ -422,7 +475,8 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
     f3 N = normalize(interp(sc.normals, id, hu, hv));
     if (dot(N, wo) < 0.0f) N = -N;
     if (rec) rec->push_back(Vertex{shader_index(sc, (uint32_t)id), 1.0f, std::vector<float>(sc.nLights, 0.0f)});
-    const f3 albedo = v3(sh.diffuseColor) * sh.diffuse;
+    f3 albedo = v3(sh.diffuseColor) * sh.diffuse;
+    if (sh.texture > 0) albedo = tex_color(sc, sh.texture, (uint32_t)id, hu, hv) * sh.diffuse;
     const f3 f = albedo * kInvPi;
     L = L + beta * v3(sh.emissionColor);
     const f3 bf = beta * f;
@@ -556,14 +610,38 @@ void orc_camera_ray(const void* cam, uint32_t w, uint32_t h, uint32_t x, uint32_
 // write float4 RGBA accumulations per listed pixel into rgba (npix*4).
 // Geometry arrays are in tree order.  opts: {mode, spp, max_depth, rr_start,
 // seed}; mode 0 = reference-compat, 1 = PBR.
+// The same with texture maps: texcoords in tree order (6 per triangle) and
+// ntex textures (descriptor {uint32 width, uint32 height, uint64 offset},
+// RGBA8 texels).
+int orc_render_tex(uint32_t count, const float* verts, const float* normals, const float* tangents,
+                   const float* bitangents, const uint16_t* shaderIds, uint32_t nShaders,
+                   const void* shaders, uint32_t nLights, const void* lights, const void* camera,
+                   uint32_t w, uint32_t h, const uint32_t* opts, const uint32_t* pixels, uint32_t npix,
+                   float* rgba, int threads, const float* texcoords, uint32_t ntex, const void* texdesc,
+                   const uint8_t* texels);
+
 int orc_render(uint32_t count, const float* verts, const float* normals, const float* tangents,
                const float* bitangents, const uint16_t* shaderIds, uint32_t nShaders,
                const void* shaders, uint32_t nLights, const void* lights, const void* camera,
                uint32_t w, uint32_t h, const uint32_t* opts, const uint32_t* pixels, uint32_t npix,
                float* rgba, int threads) {
+  return orc_render_tex(count, verts, normals, tangents, bitangents, shaderIds, nShaders, shaders, nLights,
+                        lights, camera, w, h, opts, pixels, npix, rgba, threads, nullptr, 0, nullptr, nullptr);
+}
+
+int orc_render_tex(uint32_t count, const float* verts, const float* normals, const float* tangents,
+                   const float* bitangents, const uint16_t* shaderIds, uint32_t nShaders,
+                   const void* shaders, uint32_t nLights, const void* lights, const void* camera,
+                   uint32_t w, uint32_t h, const uint32_t* opts, const uint32_t* pixels, uint32_t npix,
+                   float* rgba, int threads, const float* texcoords, uint32_t ntex, const void* texdesc,
+                   const uint8_t* texels) {
   if (count < 2 || (count & 1u) || nShaders == 0 || w == 0 || h == 0) return -1;
   Scene sc{count, verts, normals, tangents, bitangents, shaderIds, nShaders,
            (const Shader*)shaders, nLights, (const Light*)lights};
+  sc.texcoords = texcoords;
+  sc.nTextures = ntex;
+  sc.textures = (const TexDesc*)texdesc;
+  sc.texels = texels;
   const Opts o{opts[0], opts[1], opts[2], opts[3], opts[4]};
   if (o.spp == 0) return -1;
   const Camera& cam = *(const Camera*)camera;
@@ -704,6 +782,12 @@ void orc_filter_pixels(uint32_t w, uint32_t h, const uint32_t* region, const flo
 }
 
 // Exposed pieces for closed-form tests.
+void orc_tex_sample(const void* desc, const uint8_t* texels, float u, float v, float* out) {
+  const f3 c = tex_sample(*(const TexDesc*)desc, texels, u, v);
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
+}
 void orc_concentric_disk(float u1, float u2, float* out) { concentric_disk(u1, u2, out[0], out[1]); }
 void orc_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
   const u4 r = philox4x32_10(u4{ctr[0], ctr[1], ctr[2], ctr[3]}, k0, k1);

@@ -50,6 +50,10 @@ def lib():
         L.orc_transform_objects.argtypes = [u32, P, P, u32, P, P, P]
         L.orc_transform_objects.restype = i32
         L.orc_filter_pixels.argtypes = [u32, u32, P, P, P]
+        L.orc_render_tex.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, u32, P, P, u32, P, i32,
+                                     P, u32, P, P]
+        L.orc_render_tex.restype = i32
+        L.orc_tex_sample.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_philox.argtypes = [P, u32, u32, P]
         _lib = L
@@ -124,9 +128,10 @@ def camera_ray(camera: np.ndarray, w: int, h: int, x: int, y: int, sample: int, 
 
 
 def render(tree, scene, mode: int, spp: int, max_depth: int, rr_start: int, seed: int,
-           pixels: np.ndarray | None = None, threads: int = 0) -> np.ndarray:
+           pixels: np.ndarray | None = None, threads: int = 0, textures=None) -> np.ndarray:
     """tree: dict with tree-order arrays (vertices, normals, tangents,
-    bitangents, shader_ids).  Returns float32 (npix, 4) accumulations."""
+    bitangents, shader_ids, texcoords).  textures: (TEXTURE_DTYPE records,
+    uint8 texels) or None.  Returns float32 (npix, 4) accumulations."""
     opts = np.array([mode, spp, max_depth, rr_start, seed], dtype=np.uint32)
     w, h = scene.width, scene.height
     px = None if pixels is None else np.ascontiguousarray(pixels, dtype=np.uint32)
@@ -141,11 +146,17 @@ def render(tree, scene, mode: int, spp: int, max_depth: int, rr_start: int, seed
     li = np.ascontiguousarray(scene.lights)
     cam = np.ascontiguousarray(scene.camera)
     th = threads or os.cpu_count() or 1
-    rc = lib().orc_render(v.shape[0], _p(v), _p(nrm), _p(T), _p(B), _p(ids), sh.shape[0],
-                          sh.ctypes.data_as(ctypes.c_void_p), li.shape[0],
-                          li.ctypes.data_as(ctypes.c_void_p) if li.shape[0] else None,
-                          cam.ctypes.data_as(ctypes.c_void_p), w, h, _p(opts),
-                          _p(px) if px is not None else None, npix, _p(out), th)
+    uv = np.ascontiguousarray(tree["texcoords"], np.float32)
+    tdesc, texels = (None, None) if textures is None else (np.ascontiguousarray(textures[0]),
+                                                            np.ascontiguousarray(textures[1], np.uint8))
+    ntex = 0 if tdesc is None else tdesc.shape[0]
+    rc = lib().orc_render_tex(v.shape[0], _p(v), _p(nrm), _p(T), _p(B), _p(ids), sh.shape[0],
+                              sh.ctypes.data_as(ctypes.c_void_p), li.shape[0],
+                              li.ctypes.data_as(ctypes.c_void_p) if li.shape[0] else None,
+                              cam.ctypes.data_as(ctypes.c_void_p), w, h, _p(opts),
+                              _p(px) if px is not None else None, npix, _p(out), th, _p(uv), ntex,
+                              tdesc.ctypes.data_as(ctypes.c_void_p) if ntex else None,
+                              _p(texels) if ntex else None)
     if rc != 0:
         raise RuntimeError(f"orc_render failed: {rc}")
     return out
@@ -218,6 +229,14 @@ def tree_arrays(scene, perm: np.ndarray) -> dict:
         "tangents": T,
         "bitangents": B,
     }
+
+
+def tex_sample(desc, texels: np.ndarray, u: float, v: float) -> np.ndarray:
+    d = np.ascontiguousarray(desc)
+    b = np.ascontiguousarray(texels, np.uint8)
+    out = np.zeros(3, np.float32)
+    lib().orc_tex_sample(d.ctypes.data_as(ctypes.c_void_p), _p(b), u, v, _p(out))
+    return out
 
 
 def concentric_disk(u1: float, u2: float):
