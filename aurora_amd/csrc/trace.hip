@@ -34,12 +34,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                                                         const aur_ray* __restrict__ rays, uint64_t n,
                                                         aur_hit* __restrict__ hits,
                                                         int32_t* __restrict__ ids,
-                                                        float2* __restrict__ gstack, uint32_t dlo) {
+                                                        float2* __restrict__ gstack, uint32_t klds) {
   extern __shared__ float2 stk_mem[];
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const TravStack stk = dlo ? TravStack{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
-                                        (uint64_t)gridDim.x * blockDim.x, dlo, true}
-                            : lds_stack(stk_mem + threadIdx.x, blockDim.x);
+  // klds > 0: the first klds stack rows in LDS, deeper ones in global memory
+  const TravStack stk = klds ? TravStack{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
+                                         (uint64_t)gridDim.x * blockDim.x, klds, true}
+                             : lds_stack(stk_mem + threadIdx.x, blockDim.x);
   persistent_trace<kAny, kRefill>(
       TreeRef{nodes, n_nodes}, n, stk,
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef&) {
@@ -118,10 +119,12 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     AUR_LAUNCH_CHECK(ctx, "k_trace_flat");
     return AUR_OK;
   }
-  // stack rows: the deepest `rows` depths in LDS, the rest in global memory
+  // stack rows: at most levels - 1 deferred entries; the first `rows` in
+  // LDS, the rest in global memory
   const uint32_t L = ctx->levels;
-  const uint32_t rows = (tp.stack_lds_rows == 0 || tp.stack_lds_rows >= L) ? L : tp.stack_lds_rows;
-  const uint32_t dlo = L - rows;
+  const uint32_t maxrows = L > 1 ? L - 1 : 1;
+  const uint32_t rows = (tp.stack_lds_rows == 0 || tp.stack_lds_rows >= maxrows) ? maxrows : tp.stack_lds_rows;
+  const uint32_t klds = rows < maxrows ? rows : 0;
   const size_t lds_pt = (size_t)rows * block * sizeof(float2);
   // resident waves: occupancy-limited blocks per CU times CUs (or the request)
   int per_cu = 0;
@@ -142,15 +145,15 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   if (nwaves > max_waves) nwaves = max_waves;
   const uint32_t grid = (uint32_t)((nwaves * 64 + block - 1) / block);
   float2* gstack = nullptr;
-  if (dlo) {
-    AUR_HIP(ctx, ctx->gstack.reserve((size_t)dlo * grid * block));
+  if (klds) {
+    AUR_HIP(ctx, ctx->gstack.reserve((size_t)(maxrows - rows) * grid * block));
     gstack = ctx->gstack.p;
   }
   switch (tp.refill) {
-    case 1: k_trace_pt<kAny, 1><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
-    case 8: k_trace_pt<kAny, 8><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
-    case 16: k_trace_pt<kAny, 16><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
-    default: k_trace_pt<kAny, 32><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, dlo); break;
+    case 1: k_trace_pt<kAny, 1><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
+    case 8: k_trace_pt<kAny, 8><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
+    case 16: k_trace_pt<kAny, 16><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
+    default: k_trace_pt<kAny, 32><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
   }
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;

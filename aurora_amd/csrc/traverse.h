@@ -13,9 +13,10 @@
 //     exactly when the reference would pop it;
 //   * the near child is taken directly; only the far child is deferred;
 //   * a deferred child is the sibling of the current path's node at its
-//     depth, so the per-lane stack holds only its range (8 bytes in LDS,
-//     indexed by depth) plus one trail bit per depth; the child's heap index
-//     is recomputed from the last visited node when it is resumed;
+//     depth, so the per-lane stack holds only its range (8 bytes, a
+//     compact LIFO stack in depth order) plus one trail bit per depth; the
+//     child's heap index is recomputed from the last visited node when it is
+//     resumed;
 //   * the slab quotient (s - p) / d uses the ray's correctly rounded
 //     reciprocal y = RN(1/d) and one residual correction,
 //         q0 = RN(n y),  e = n - d q0 (exact, FMA),  q = RN(q0 + e y),
@@ -92,28 +93,29 @@ struct Trav {
 
 __device__ __forceinline__ float compf(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
-// Per-lane stack of deferred ranges, one row per tree depth.  Rows for depth
-// >= dlo live in LDS (lane-strided, conflict-free); shallower rows live in a
-// per-lane column of global memory.  Deep entries are pushed and popped at
-// almost every visit, shallow ones a few times per ray, so keeping only the
-// deep rows in LDS frees LDS for more resident waves.
-// Measured at 1M triangles the split is slower than keeping every row in LDS
-// (occupancy is not what limits the kernel), so it is a tuning option only.
+// Per-lane stack of deferred ranges.  A deferred child is always deeper
+// than every entry already pending (entries sit on the current path's
+// siblings and are popped deepest first), so the entries form a LIFO stack
+// sorted by depth and entry i is at index i = popcount(trail) at push time:
+// at most levels - 1 entries, indices <= levels - 2.  Rows [0, klds) live in
+// LDS (lane-strided, conflict-free); deeper rows spill to a per-lane column
+// of global memory, so a small LDS window leaves room for more resident
+// waves.
 struct TravStack {
   float2* lds;        // row r of this lane at lds[r * lstride]
   uint32_t lstride;
-  float2* glb;        // row d of this lane at glb[d * gstride]; null: all rows in LDS
+  float2* glb;        // row r >= klds at glb[(r - klds) * gstride]; unused without split
   uint64_t gstride;
-  uint32_t dlo;
+  uint32_t klds;
   bool split;
-  __device__ __forceinline__ void store(uint32_t d, float2 v) const {
-    if (!split || d >= dlo)
-      lds[(d - dlo) * lstride] = v;
+  __device__ __forceinline__ void store(uint32_t r, float2 v) const {
+    if (!split || r < klds)
+      lds[r * lstride] = v;
     else
-      glb[d * gstride] = v;
+      glb[(r - klds) * gstride] = v;
   }
-  __device__ __forceinline__ float2 load(uint32_t d) const {
-    return (!split || d >= dlo) ? lds[(d - dlo) * lstride] : glb[d * gstride];
+  __device__ __forceinline__ float2 load(uint32_t r) const {
+    return (!split || r < klds) ? lds[r * lstride] : glb[(r - klds) * gstride];
   }
 };
 
@@ -208,7 +210,7 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     const bool take_near = nok && near_r.x <= s.t;
     const bool take_far = !take_near && fok && far_r.x <= s.t;
     if (take_near && fok) {
-      stk.store(cd, far_r);
+      stk.store((uint32_t)__popc(s.trail), far_r);
       s.trail |= 1u << cd;
     }
     found = take_near || take_far;
@@ -220,8 +222,8 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   if (!stop && !found) {
     while (s.trail) {
       const uint32_t dd = 31u - (uint32_t)__clz(s.trail);
+      const float2 fr = stk.load((uint32_t)__popc(s.trail) - 1u);
       s.trail &= ~(1u << dd);
-      const float2 fr = stk.load(dd);
       if (fr.x <= s.t) {
         const uint32_t anc = ((k + 1) >> (s.depth - dd)) - 1;
         nk = (anc & 1u) ? anc + 1 : anc - 1;

@@ -162,8 +162,9 @@ aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int3
  * waves each own a contiguous ray range and refill idle lanes once `refill`
  * (1, 8, 16 or 32) lanes are idle; waves_per_cu = 0 uses the occupancy
  * limit.  persistent = 0: one lane per ray.  block: 64, 128 or 256.
- * stack_lds_rows: how many of the deepest tree levels keep each lane's
- * deferred ranges in LDS (0 = all); shallower levels use global memory.
+ * stack_lds_rows: how many rows of each lane's stack of deferred ranges
+ * (a compact stack of at most levels - 1 entries) live in LDS (0 = all);
+ * deeper entries spill to global memory, leaving LDS for more waves.
  * Results do not depend on these values. */
 typedef struct aur_trace_params {
   uint32_t persistent, block, refill, waves_per_cu, stack_lds_rows;
