@@ -265,6 +265,12 @@ aur_status aur_trace_closest_stats(aur_context* ctx, uint64_t n, const aur_ray* 
   return aur::trace_closest_count(ctx, n, rays, hits, popped, passed);
 }
 
+aur_status aur_check_division(aur_context* ctx, uint64_t samples, uint64_t seed,
+                              uint64_t* recip_checked, uint64_t* recip_bad, uint64_t* quot_bad) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  return aur::division_check(ctx, samples, seed, recip_checked, recip_bad, quot_bad);
+}
+
 aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* hit_ids) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   return aur::trace_any(ctx, n, rays, hit_ids);

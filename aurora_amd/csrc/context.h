@@ -187,6 +187,8 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats);
 aur_status generate_tb(aur_context* ctx);
 aur_status trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits);
 aur_status trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int32_t* ids);
+aur_status division_check(aur_context* ctx, uint64_t samples, uint64_t seed, uint64_t* recip_checked,
+                          uint64_t* recip_bad, uint64_t* quot_bad);
 aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
                                uint64_t* popped, uint64_t* passed);
 aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,

@@ -3,8 +3,8 @@
 // k_trace_pt: persistent waves (persistent_trace in traverse.h): each wave
 // owns a contiguous range of rays and refills idle lanes from it, so lanes
 // whose rays finish early do not idle until the slowest ray of the wave ends.
-// Each lane keeps its deferred ranges in an LDS column of `levels` float2
-// entries.
+// Each lane keeps its deferred ranges in an LDS column of levels - 1 float2
+// entries (or the first rows in LDS and the rest in global memory).
 //
 // k_trace_flat: one lane per ray, no refill (reference schedule; used for
 // comparison and by the instrumented counting launch).
@@ -28,7 +28,7 @@ __device__ __forceinline__ void store_result(uint64_t i, const Trav& s, aur_hit*
     reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(s.id), s.t, s.u, s.v);
 }
 
-template <bool kAny, int kRefill>
+template <bool kAny, int kRefill, bool kSplit>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restrict__ nodes,
                                                         uint32_t n_nodes,
                                                         const aur_ray* __restrict__ rays, uint64_t n,
@@ -36,22 +36,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                                                         int32_t* __restrict__ ids,
                                                         float2* __restrict__ gstack, uint32_t klds) {
   extern __shared__ float2 stk_mem[];
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // klds > 0: the first klds stack rows in LDS, deeper ones in global memory
-  const TravStack stk = klds ? TravStack{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
-                                         (uint64_t)gridDim.x * blockDim.x, klds, true}
-                             : lds_stack(stk_mem + threadIdx.x, blockDim.x);
-  persistent_trace<kAny, kRefill>(
-      TreeRef{nodes, n_nodes}, n, stk,
-      [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef&) {
-        const float4* r = reinterpret_cast<const float4*>(rays + i);
-        const float4 a = r[0], b = r[1];
-        o = mk(a.x, a.y, a.z);
-        d = mk(b.x, b.y, b.z);
-        tmax = a.w;
-        return true;
-      },
-      [&](uint64_t i, const Trav& st) { store_result(i, st, hits, ids); });
+  const auto fetch = [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef&) {
+    const float4* r = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = r[0], b = r[1];
+    o = mk(a.x, a.y, a.z);
+    d = mk(b.x, b.y, b.z);
+    tmax = a.w;
+    return true;
+  };
+  const auto store = [&](uint64_t i, const Trav& st) { store_result(i, st, hits, ids); };
+  if constexpr (kSplit) {
+    // the first klds stack rows in LDS, deeper ones in global memory
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const SplitStack stk{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
+                         (uint64_t)gridDim.x * blockDim.x, klds};
+    persistent_trace<kAny, kRefill>(TreeRef{nodes, n_nodes}, n, stk, fetch, store);
+  } else {
+    persistent_trace<kAny, kRefill>(TreeRef{nodes, n_nodes}, n,
+                                    lds_stack(stk_mem + threadIdx.x, blockDim.x), fetch, store);
+  }
 }
 
 template <bool kAny>
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restr
   const float4 a = r[0], b = r[1];
   Trav s;
   if (trav_init<false>(s, nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
-    const TravStack stk = lds_stack(stk_mem + threadIdx.x, blockDim.x);
+    const LdsStack stk = lds_stack(stk_mem + threadIdx.x, blockDim.x);
     while (!trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
     }
   }
@@ -105,6 +108,50 @@ __global__ void k_zero2(unsigned long long* p) {
   if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
 }
 
+// Self-check of the division shortcuts used by the traversal:
+//   sums[0] += mismatches of recip_rn(x) against 1.0f / x over every float x
+//              in [2^-24, 2^64] (bit patterns lo..hi);
+//   sums[1] += mismatches of quot_fast(n, d, RN(1/d)) against n / d over
+//              `samples` hashed pairs with 2^-60 <= |d| < 16 and n = 0 or
+//              2^-96 <= |n| < 2^60, random signs and mantissas.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_division_check(uint32_t lo, uint32_t hi, uint64_t samples,
+                                                        uint64_t seed,
+                                                        unsigned long long* __restrict__ sums) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bad_r = 0, bad_q = 0;
+  for (uint64_t b = lo + gtid; b <= hi; b += stride) {
+    const float x = __uint_as_float((uint32_t)b);
+    bad_r += __float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x);
+  }
+  for (uint64_t i = gtid; i < samples; i += stride) {
+    const uint64_t h = mix64(seed ^ mix64(i));
+    const uint32_t md = (uint32_t)h & 0x7FFFFFu, mn = (uint32_t)(h >> 23) & 0x7FFFFFu;
+    const uint32_t ed = 127u - 60u + (uint32_t)((h >> 46) % 64u);   // 2^-60 .. 2^3
+    const uint32_t en = 127u - 96u + (uint32_t)((h >> 52) % 156u);  // 2^-96 .. 2^59
+    const uint32_t sd = (uint32_t)(h >> 62) & 1u, sn = (uint32_t)(h >> 63);
+    const float d = __uint_as_float((sd << 31) | (ed << 23) | md);
+    const float n = (i % 1024u == 7u) ? 0.0f : __uint_as_float((sn << 31) | (en << 23) | mn);
+    const float y = 1.0f / d;
+    bad_q += __float_as_uint(quot_fast(n, d, y)) != __float_as_uint(n / d);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    bad_r += __shfl_xor(bad_r, off);
+    bad_q += __shfl_xor(bad_q, off);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    if (bad_r) atomicAdd(&sums[0], (unsigned long long)bad_r);
+    if (bad_q) atomicAdd(&sums[1], (unsigned long long)bad_q);
+  }
+}
+
 template <bool kAny>
 aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
                         int32_t* ids) {
@@ -128,15 +175,18 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   const size_t lds_pt = (size_t)rows * block * sizeof(float2);
   // resident waves: occupancy-limited blocks per CU times CUs (or the request)
   int per_cu = 0;
-  const void* fn = nullptr;
+  using Kern = void (*)(const NodeRec*, uint32_t, const aur_ray*, uint64_t, aur_hit*, int32_t*,
+                        float2*, uint32_t);
+  Kern fn = nullptr;
+  const bool split = klds != 0;
   switch (tp.refill) {
-    case 1: fn = (const void*)k_trace_pt<kAny, 1>; break;
-    case 8: fn = (const void*)k_trace_pt<kAny, 8>; break;
-    case 16: fn = (const void*)k_trace_pt<kAny, 16>; break;
-    case 32: fn = (const void*)k_trace_pt<kAny, 32>; break;
+    case 1: fn = split ? k_trace_pt<kAny, 1, true> : k_trace_pt<kAny, 1, false>; break;
+    case 8: fn = split ? k_trace_pt<kAny, 8, true> : k_trace_pt<kAny, 8, false>; break;
+    case 16: fn = split ? k_trace_pt<kAny, 16, true> : k_trace_pt<kAny, 16, false>; break;
+    case 32: fn = split ? k_trace_pt<kAny, 32, true> : k_trace_pt<kAny, 32, false>; break;
     default: return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be 1, 8, 16 or 32");
   }
-  AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds_pt));
+  AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, (int)block, lds_pt));
   if (per_cu < 1) per_cu = 1;
   uint64_t blocks_total = (uint64_t)per_cu * ctx->num_cus;
   if (tp.waves_per_cu) blocks_total = std::max<uint64_t>(1, (uint64_t)tp.waves_per_cu * ctx->num_cus * 64 / block);
@@ -149,12 +199,8 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     AUR_HIP(ctx, ctx->gstack.reserve((size_t)(maxrows - rows) * grid * block));
     gstack = ctx->gstack.p;
   }
-  switch (tp.refill) {
-    case 1: k_trace_pt<kAny, 1><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
-    case 8: k_trace_pt<kAny, 8><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
-    case 16: k_trace_pt<kAny, 16><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
-    default: k_trace_pt<kAny, 32><<<grid, block, lds_pt, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n, hits, ids, gstack, klds); break;
-  }
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds_pt, ctx->stream, ctx->nodes.p, ctx->count / 2,
+                     rays, n, hits, ids, gstack, klds);
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;
 }
@@ -182,6 +228,23 @@ aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays
   AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (popped) *popped = h[0];
   if (passed) *passed = h[1];
+  return AUR_OK;
+}
+
+aur_status division_check(aur_context* ctx, uint64_t samples, uint64_t seed, uint64_t* recip_checked,
+                          uint64_t* recip_bad, uint64_t* quot_bad) {
+  DevBuf<unsigned long long> sums;
+  AUR_HIP(ctx, sums.reserve(2));
+  k_zero2<<<1, 64, 0, ctx->stream>>>(sums.p);
+  const uint32_t lo = 0x33800000u, hi = 0x5F800000u;  // 2^-24 .. 2^64
+  k_division_check<<<4096, 256, 0, ctx->stream>>>(lo, hi, samples, seed, sums.p);
+  AUR_LAUNCH_CHECK(ctx, "k_division_check");
+  unsigned long long h[2] = {0, 0};
+  AUR_HIP(ctx, hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (recip_checked) *recip_checked = (uint64_t)(hi - lo) + 1;
+  if (recip_bad) *recip_bad = h[0];
+  if (quot_bad) *quot_bad = h[1];
   return AUR_OK;
 }
 

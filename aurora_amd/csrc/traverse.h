@@ -22,8 +22,13 @@
 //         q0 = RN(n y),  e = n - d q0 (exact, FMA),  q = RN(q0 + e y),
 //     which equals RN(n / d) (Markstein's theorem) while no step under- or
 //     overflows: used only for 2^-60 <= |d| <= 16 and n = 0 or
-//     2^-96 <= |n| <= 2^60, true division otherwise.
-// Triangle tests keep the correctly rounded 1/det of the reference.
+//     2^-96 <= |n| <= 2^60, true division otherwise;
+//   * both triangles of a node are tested without branches (every value is
+//     computed, the reference's early outs become one accept predicate with
+//     the same NaN behaviour), and 1/det is HIP's correctly rounded
+//     reciprocal sequence without its scaling and fix-up steps, which are
+//     identities for 2^-24 <= det <= 2^64 (det < kEpsilon is rejected before
+//     the value is used; det > 2^64 takes true division).
 //
 // The traversal is a small state machine (trav_init / trav_step, one node
 // visit per step) so that persistent kernels can refill finished lanes.
@@ -33,40 +38,75 @@
 
 namespace aur {
 
+// Bitwise (not short-circuit) conditions keep these tests free of branches.
 __device__ __forceinline__ bool fast_num(float n) {
   const float a = fabsf(n);
-  return n == 0.0f || (a >= 0x1p-96f && a <= 0x1p60f);
+  return (n == 0.0f) | ((a >= 0x1p-96f) & (a <= 0x1p60f));
+}
+
+// RN(n / d) by the reciprocal quotient (valid when fast_num(n) and fast_d).
+__device__ __forceinline__ float quot_fast(float n, float d, float y) {
+  const float q = n * y;
+  return __builtin_fmaf(__builtin_fmaf(-d, q, n), y, q);
+}
+
+__device__ __forceinline__ float2 clip(float2 r, float d0, float d1) {
+  return make_float2(fmaxf(fminf(d0, d1), r.x), fminf(fmaxf(d0, d1), r.y));
 }
 
 __device__ __forceinline__ float2 narrow(float2 r, float slo, float shi, float p, float d, float y,
                                          bool fast_d) {
   const float n0 = slo - p, n1 = shi - p;
-  float d0, d1;
-  if (fast_d && fast_num(n0) && fast_num(n1)) {
-    const float q0 = n0 * y, q1 = n1 * y;
-    d0 = __builtin_fmaf(__builtin_fmaf(-d, q0, n0), y, q0);
-    d1 = __builtin_fmaf(__builtin_fmaf(-d, q1, n1), y, q1);
-  } else {
+  float d0 = quot_fast(n0, d, y), d1 = quot_fast(n1, d, y);
+  const bool f0 = fast_num(n0), f1 = fast_num(n1);
+  if (!(fast_d & f0 & f1)) {
     d0 = n0 / d;
     d1 = n1 / d;
   }
-  return make_float2(fmaxf(fminf(d0, d1), r.x), fminf(fmaxf(d0, d1), r.y));
+  return clip(r, d0, d1);
 }
 
-__device__ __forceinline__ bool tri_test(V3 o, V3 d, V3 v1, V3 e1, V3 e2, float& u, float& v,
-                                         float& t) {
-  const V3 P = cross(d, e2);
-  const float det = dot(e1, P);
-  if (det < kEpsilon) return false;
-  const float invdet = 1.0f / det;
+// Both children's slabs on one axis: one shared (rarely taken) branch to
+// true division.
+__device__ __forceinline__ void narrow2(float2 r, float4 slabs, float p, float d, float y,
+                                        bool fast_d, float2& rl, float2& rr) {
+  const float n0 = slabs.x - p, n1 = slabs.y - p, n2 = slabs.z - p, n3 = slabs.w - p;
+  float d0 = quot_fast(n0, d, y), d1 = quot_fast(n1, d, y);
+  float d2 = quot_fast(n2, d, y), d3 = quot_fast(n3, d, y);
+  const bool f0 = fast_num(n0), f1 = fast_num(n1), f2 = fast_num(n2), f3 = fast_num(n3);
+  if (!(fast_d & f0 & f1 & f2 & f3)) {
+    d0 = n0 / d;
+    d1 = n1 / d;
+    d2 = n2 / d;
+    d3 = n3 / d;
+  }
+  rl = clip(r, d0, d1);
+  rr = clip(r, d2, d3);
+}
+
+// RN(1/x) for 2^-24 <= x <= 2^64: the steps of HIP's IEEE division 1/x
+// (reciprocal estimate, one Newton step, quotient and two residual
+// corrections) without the operand scaling and result fix-up, which do
+// nothing on this range.
+constexpr float kRecipMax = 0x1p64f;
+__device__ __forceinline__ float recip_rn(float x) {
+  const float r0 = __builtin_amdgcn_rcpf(x);
+  const float r1 = __builtin_fmaf(__builtin_fmaf(-x, r0, 1.0f), r0, r0);
+  const float q1 = __builtin_fmaf(__builtin_fmaf(-x, r1, 1.0f), r1, r1);
+  return __builtin_fmaf(__builtin_fmaf(-x, q1, 1.0f), r1, q1);
+}
+
+// The reference's one-sided Moller-Trumbore test after its determinant:
+// accept iff !(det < eps), u in [0,1], v >= 0 and u + v <= 1 (NaN passes
+// these as in the reference and is rejected by the caller's t test).
+__device__ __forceinline__ bool tri_finish(V3 o, V3 d, V3 v1, V3 e1, V3 e2, V3 P, float det,
+                                           float invdet, float& u, float& v, float& t) {
   const V3 T = o - v1;
   u = dot(T, P) * invdet;
-  if (u < 0.0f || u > 1.0f) return false;
   const V3 Q = cross(T, e1);
   v = dot(d, Q) * invdet;
-  if (v < 0.0f || u + v > 1.0f) return false;
   t = dot(e2, Q) * invdet;
-  return true;
+  return !(det < kEpsilon) & !(u < 0.0f) & !(u > 1.0f) & !(v < 0.0f) & !(u + v > 1.0f);
 }
 
 struct TraceResult {
@@ -97,31 +137,38 @@ __device__ __forceinline__ float compf(V3 v, int a) { return a == 0 ? v.x : (a =
 // than every entry already pending (entries sit on the current path's
 // siblings and are popped deepest first), so the entries form a LIFO stack
 // sorted by depth and entry i is at index i = popcount(trail) at push time:
-// at most levels - 1 entries, indices <= levels - 2.  Rows [0, klds) live in
-// LDS (lane-strided, conflict-free); deeper rows spill to a per-lane column
-// of global memory, so a small LDS window leaves room for more resident
-// waves.
-struct TravStack {
-  float2* lds;        // row r of this lane at lds[r * lstride]
+// at most levels - 1 entries, indices <= levels - 2.
+//
+// LdsStack: every row in LDS (lane-strided, conflict-free).
+struct LdsStack {
+  float2* lds;  // row r of this lane at lds[r * stride]
+  uint32_t stride;
+  __device__ __forceinline__ void store(uint32_t r, float2 v) const { lds[r * stride] = v; }
+  __device__ __forceinline__ float2 load(uint32_t r) const { return lds[r * stride]; }
+};
+
+// SplitStack: rows [0, klds) in LDS, deeper rows in a per-lane column of
+// global memory, so that a small LDS window leaves room for more resident
+// waves.  (Its accesses go through generic pointers.)
+struct SplitStack {
+  float2* lds;
   uint32_t lstride;
-  float2* glb;        // row r >= klds at glb[(r - klds) * gstride]; unused without split
+  float2* glb;  // row r >= klds at glb[(r - klds) * gstride]
   uint64_t gstride;
   uint32_t klds;
-  bool split;
   __device__ __forceinline__ void store(uint32_t r, float2 v) const {
-    if (!split || r < klds)
+    if (r < klds)
       lds[r * lstride] = v;
     else
       glb[(r - klds) * gstride] = v;
   }
   __device__ __forceinline__ float2 load(uint32_t r) const {
-    return (!split || r < klds) ? lds[r * lstride] : glb[(r - klds) * gstride];
+    return r < klds ? lds[r * lstride] : glb[(r - klds) * gstride];
   }
 };
 
-// All rows in LDS.
-__device__ __forceinline__ TravStack lds_stack(float2* lane_column, uint32_t stride) {
-  return TravStack{lane_column, stride, nullptr, 0, 0, false};
+__device__ __forceinline__ LdsStack lds_stack(float2* lane_column, uint32_t stride) {
+  return LdsStack{lane_column, stride};
 }
 
 // Sets up a ray; returns false when the root's slab culls it (finished, miss).
@@ -153,10 +200,9 @@ __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ n
 // the next node is chosen by selects: with several early exits the compiler's
 // structured control flow ran the node visit separately for lanes arriving
 // from different paths, doubling the instructions issued per visit.
-template <bool kAny, bool kCount>
+template <bool kAny, bool kCount, class Stk>
 __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ nodes,
-                                          uint32_t n_nodes, const TravStack& stk,
-                                          TraceCount* cnt) {
+                                          uint32_t n_nodes, const Stk& stk, TraceCount* cnt) {
   const uint32_t k = s.k;
   if (kCount) {
     cnt->passed += 1;
@@ -168,25 +214,34 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   const float4* rec = reinterpret_cast<const float4*>(nodes + k);
   const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
   const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
-  float u, v, t;
-  bool stop = false;  // any hit found
-  if (tri_test(s.o, s.d, mk(a0.x, a0.y, a0.z), mk(a0.w, a1.x, a1.y), mk(a1.z, a1.w, a2.x), u, v, t) &&
-      t > kEpsilon && t < s.t) {
-    s.t = t;
-    s.u = u;
-    s.v = v;
+  // both triangles up to their determinants, then both reciprocals
+  const V3 v1a = mk(a0.x, a0.y, a0.z), e1a = mk(a0.w, a1.x, a1.y), e2a = mk(a1.z, a1.w, a2.x);
+  const V3 v1b = mk(a2.y, a2.z, a2.w), e1b = mk(a3.x, a3.y, a3.z), e2b = mk(a3.w, a4.x, a4.y);
+  const V3 Pa = cross(s.d, e2a), Pb = cross(s.d, e2b);
+  const float deta = dot(e1a, Pa), detb = dot(e1b, Pb);
+  float ia = recip_rn(deta), ib = recip_rn(detb);
+  if ((deta > kRecipMax) | (detb > kRecipMax)) {
+    ia = deta > kRecipMax ? 1.0f / deta : ia;
+    ib = detb > kRecipMax ? 1.0f / detb : ib;
+  }
+  float ua, va, ta, ub, vb, tb;
+  const bool hita = tri_finish(s.o, s.d, v1a, e1a, e2a, Pa, deta, ia, ua, va, ta) &
+                    (ta > kEpsilon) & (ta < s.t);
+  if (hita) {
+    s.t = ta;
+    s.u = ua;
+    s.v = va;
     s.id = (int32_t)(2 * k);
-    if (kAny) stop = true;
   }
-  if (!stop &&
-      tri_test(s.o, s.d, mk(a2.y, a2.z, a2.w), mk(a3.x, a3.y, a3.z), mk(a3.w, a4.x, a4.y), u, v, t) &&
-      t > kEpsilon && t < s.t) {
-    s.t = t;
-    s.u = u;
-    s.v = v;
+  const bool hitb = tri_finish(s.o, s.d, v1b, e1b, e2b, Pb, detb, ib, ub, vb, tb) &
+                    (tb > kEpsilon) & (tb < s.t) & !(kAny & hita);
+  if (hitb) {
+    s.t = tb;
+    s.u = ub;
+    s.v = vb;
     s.id = (int32_t)(2 * k + 1);
-    if (kAny) stop = true;
   }
+  const bool stop = kAny & (hita | hitb);  // any hit found
   bool found = false;
   uint32_t nk = 0, nd = 0, na = 0;
   float2 nrg = s.r;
@@ -196,14 +251,10 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     const int ca = s.axis == 2 ? 0 : (int)s.axis + 1;
     const float oc = compf(s.o, ca), dc = compf(s.d, ca), yc = compf(s.y, ca);
     const bool fd = (s.fast >> ca) & 1u;
-    const float2 rl = narrow(s.r, a4.z, a4.w, oc, dc, yc, fd);
+    float2 rl, rr;
+    narrow2(s.r, make_float4(a4.z, a4.w, cs.x, cs.y), oc, dc, yc, fd, rl, rr);
     const bool okl = rl.x <= rl.y;
-    float2 rr = make_float2(0.0f, 0.0f);
-    bool okr = false;
-    if (cl + 1 < n_nodes) {
-      rr = narrow(s.r, cs.x, cs.y, oc, dc, yc, fd);
-      okr = rr.x <= rr.y;
-    }
+    const bool okr = (cl + 1 < n_nodes) & (rr.x <= rr.y);
     const bool sw = dc < 0.0f;
     const float2 near_r = sw ? rr : rl, far_r = sw ? rl : rr;
     const bool nok = sw ? okr : okl, fok = sw ? okl : okr;
@@ -257,8 +308,8 @@ struct TreeRef {
 // state.  A fetch that never writes `tree` keeps the tree wave-uniform.
 // (Splitting a refill over two iterations, so that the item loads overlap
 // the active lanes' node step, measured slower: the new lanes lose a step.)
-template <bool kAny, int kRefill, class Fetch, class Store>
-__device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, const TravStack& stk,
+template <bool kAny, int kRefill, class Stk, class Fetch, class Store>
+__device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, const Stk& stk,
                                                  Fetch fetch, Store store) {
   TreeRef tree = tree0;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -307,7 +358,7 @@ __device__ __forceinline__ TraceResult trace_ray(const NodeRec* __restrict__ nod
                                                  V3 o, V3 d, float tmax, float2* stk,
                                                  uint32_t stride, TraceCount* cnt = nullptr) {
   Trav s;
-  const TravStack st = lds_stack(stk, stride);
+  const LdsStack st = lds_stack(stk, stride);
   if (trav_init<kCount>(s, nodes, o, d, tmax, cnt)) {
     while (!trav_step<kAny, kCount>(s, nodes, n_nodes, st, cnt)) {
     }

@@ -108,6 +108,8 @@ def load():
         "aur_trace_closest": ([P, u64, P, P], i32),
         "aur_trace_any": ([P, u64, P, P], i32),
         "aur_trace_closest_stats": ([P, u64, P, P, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
+        "aur_check_division": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                ctypes.POINTER(u64)], i32),
         "aur_set_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
         "aur_get_trace_params": ([P, ctypes.POINTER(TraceParams)], i32),
         "aur_set_render_params": ([P, ctypes.POINTER(RenderParams)], i32),
@@ -298,6 +300,15 @@ class Context:
                                                       ctypes.c_void_p(hits_ptr), ctypes.byref(p),
                                                       ctypes.byref(q)), "trace_closest_stats")
         return int(p.value), int(q.value)
+
+    def check_division(self, samples: int, seed: int = 1) -> tuple[int, int, int]:
+        """(floats checked, reciprocal mismatches, quotient mismatches) of the
+        traversal's division shortcuts against true division, on the device."""
+        c, r, q = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._lib.aur_check_division(self._h, samples, seed, ctypes.byref(c),
+                                                 ctypes.byref(r), ctypes.byref(q)),
+                    "check_division")
+        return int(c.value), int(r.value), int(q.value)
 
     def trace_any_ptr(self, n: int, rays_ptr: int, ids_ptr: int) -> None:
         self._check(self._lib.aur_trace_any(self._h, n, ctypes.c_void_p(rays_ptr),

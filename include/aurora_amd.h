@@ -177,6 +177,14 @@ aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* params);
  * 32 + 16 + 24*popped + 48*passed bytes per ray. */
 aur_status aur_trace_closest_stats(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
                                    uint64_t* popped, uint64_t* passed);
+/* Device self-check of the traversal's division shortcuts (no reference
+ * counterpart: the reference divides directly).  recip_bad counts floats x in
+ * [2^-24, 2^64] (all recip_checked of them) whose shortcut reciprocal differs
+ * from 1.0f / x; quot_bad counts, over `samples` hashed pairs (n, d) in the
+ * shortcut's range, reciprocal quotients that differ from n / d.  Both must be
+ * 0.  Synchronous. */
+aur_status aur_check_division(aur_context* ctx, uint64_t samples, uint64_t seed,
+                              uint64_t* recip_checked, uint64_t* recip_bad, uint64_t* quot_bad);
 
 /* ---- frame (replaces MonteCarloRaytracer::createFrame/destroyFrame/
  * setRegion/render/framebuffer, src/render/monteCarloRaytracer.cpp:23-127,

@@ -125,6 +125,36 @@ def test_trace_soup_with_ties(gpu_ctx, oracle):
     assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32))
 
 
+def test_division_shortcuts_exact(gpu_ctx):
+    """The traversal's reciprocal shortcut equals 1.0f/x for every float in
+    [2^-24, 2^64]; its slab quotient equals n/d on 2^30 sampled pairs."""
+    checked, recip_bad, quot_bad = gpu_ctx.check_division(1 << 30, seed=5)
+    assert checked == 0x5F800000 - 0x33800000 + 1
+    assert recip_bad == 0
+    assert quot_bad == 0
+
+
+@pytest.mark.parametrize("scale,dir_scale,hits", [(1e5, 1e11, True), (1e-30, 1e-30, False),
+                                                   (1.0, 100.0, True), (1e-15, 1e-19, False)])
+def test_trace_far_from_unit_scale(gpu_ctx, oracle, scale, dir_scale, hits):
+    """Scenes and rays outside the shortcuts' ranges take true division and
+    still equal the oracle bit for bit: determinants above 2^64 with
+    |direction| above 16; slab numerators below 2^-96; |direction| below
+    2^-60.  (The reference's absolute det and t thresholds reject every hit
+    of the tiny scenes.)"""
+    s = scenes.random_soup(seed=22, n=20_000, box=10.0)
+    s.vertices = (s.vertices.astype(np.float64) * scale).astype(np.float32)
+    rays = scenes.incoherent_rays(50_000, 0.0, 10.0, seed=16)
+    rays[:, 0:3] = (rays[:, 0:3].astype(np.float64) * scale).astype(np.float32)
+    rays[:, 4:7] = (rays[:, 4:7].astype(np.float64) * dir_scale).astype(np.float32)
+    rays[:, 3] = np.float32(1e30)
+    ids_g, tuv_g, ids_o, tuv_o, _ = _trace_both(gpu_ctx, oracle, s, rays)
+    assert np.array_equal(ids_g, ids_o)
+    assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32))
+    if hits:
+        assert (ids_g >= 0).sum() > 1000
+
+
 # --- rendering: per-pixel replay --------------------------------------------
 
 
