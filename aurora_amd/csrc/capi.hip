@@ -56,8 +56,8 @@ aur_status aur_set_trace_params(aur_context* ctx, const aur_trace_params* p) {
   if (!p) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: null");
   if (p->block != 64 && p->block != 128 && p->block != 256)
     return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: block must be 64, 128 or 256");
-  if (p->persistent && p->refill != 1 && p->refill != 8 && p->refill != 16 && p->refill != 32)
-    return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: refill must be 1, 8, 16 or 32");
+  if (p->persistent && (p->refill < 1 || p->refill > 64))
+    return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: refill must be in 1..64");
   if (p->waves_per_cu > 64) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: waves_per_cu > 64");
   if (p->stack_lds_rows > 32) return ctx->fail(AUR_ERR_INVALID_ARG, "trace params: stack_lds_rows > 32");
   ctx->trace_params = *p;

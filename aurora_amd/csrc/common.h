@@ -139,11 +139,24 @@ constexpr uint32_t kTagPbr = 0x50425200u;
 // e2 = v3-v1) -- the same float values Ray::intersect computes on the fly
 // (src/kernels/lib/ray.cuh:25-26) -- followed by the slabs of the node's two
 // children on the children's axis (min/max of their six axis coordinates,
-// src/kernels/lib/intersect.cuh:106-111).  96 bytes, 16-byte aligned.
+// src/kernels/lib/intersect.cuh:106-111), and flags.  96 bytes, 16-byte
+// aligned.  One extra record after the last node (index n) holds the root's
+// own slab (axis 0) in cslab[0..1].
+constexpr uint32_t kNodeSlabsNormal = 1u;  // every cslab value passes normal_coord
+
+// Coordinates x = 0 or 2^-72 <= |x| <= 2^59.  For a slab value s and a ray
+// origin coordinate p that both pass, the numerator n = RN(s - p) is 0 or
+// 2^-96 <= |n| <= 2^60: |s - p| <= 2^60, and a nonzero difference of two such
+// floats is a multiple of 2^(-72-23).
+__host__ __device__ inline bool normal_coord(float x) {
+  const float a = fabsf(x);
+  return (x == 0.0f) | ((a >= 0x1p-72f) & (a <= 0x1p59f));
+}
 struct alignas(16) NodeRec {
   float tri[2][9];
   float cslab[4];  // left lo, left hi, right lo, right hi
-  float pad[2];
+  uint32_t flags;
+  uint32_t spare;
 };
 static_assert(sizeof(NodeRec) == 96, "node record");
 

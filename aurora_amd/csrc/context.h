@@ -121,9 +121,9 @@ struct aur_context {
   hipStream_t stream = nullptr;
   std::string last_error;
   uint32_t num_cus = 256;
-  // persistent waves, 256-thread blocks, refill at 16 idle lanes, occupancy
+  // persistent waves, 256-thread blocks, refill at 24 idle lanes, occupancy
   // limited residency: fastest shape measured at 1M triangles / 2^24 rays
-  aur_trace_params trace_params{1, 256, 16, 0, 0};
+  aur_trace_params trace_params{1, 256, 24, 0, 0};
   aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
   aur_render_params render_params{aur::kDefaultMaxPaths, 0};
 

@@ -448,8 +448,18 @@ __device__ inline void slab_of(const float* v, int a, float& lo, float& hi) {
 __global__ void k_node_records(const float* __restrict__ v, uint32_t n,
                                NodeRec* __restrict__ nodes) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k > n) return;
   NodeRec r;
+  if (k == n) {  // the root's own slab, on axis 0
+#pragma unroll
+    for (int t = 0; t < 18; ++t) (&r.tri[0][0])[t] = 0.0f;
+    slab_of(v, 0, r.cslab[0], r.cslab[1]);
+    r.cslab[2] = r.cslab[3] = 0.0f;
+    r.flags = 0;
+    r.spare = 0;
+    nodes[k] = r;
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const float* p = v + (size_t)(2 * k + t) * 9;  // points layout
@@ -472,7 +482,11 @@ __global__ void k_node_records(const float* __restrict__ v, uint32_t n,
   const uint64_t cl = 2ull * k + 1;
   if (cl < n) slab_of(v + cl * 18, ca, r.cslab[0], r.cslab[1]);
   if (cl + 1 < n) slab_of(v + (cl + 1) * 18, ca, r.cslab[2], r.cslab[3]);
-  slab_of(v + (size_t)k * 18, (int)(d % 3), r.pad[0], r.pad[1]);
+  bool normal = true;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) normal = normal && normal_coord(r.cslab[c]);
+  r.flags = normal ? kNodeSlabsNormal : 0u;
+  r.spare = 0;
   nodes[k] = r;
 }
 
@@ -693,13 +707,13 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, ctx->normals.reserve((size_t)N * 9));
   AUR_HIP(ctx, ctx->texcoords.reserve((size_t)N * 6));
   AUR_HIP(ctx, ctx->shader_ids.reserve(N));
-  AUR_HIP(ctx, ctx->nodes.reserve(n));
+  AUR_HIP(ctx, ctx->nodes.reserve((size_t)n + 1));  // + the root's own slab
   k_gather_geometry<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N, ctx->in_vertices.p,
                                                    ctx->in_normals.p, ctx->in_texcoords.p,
                                                    ctx->in_shader_ids.p, ctx->vertices.p,
                                                    ctx->normals.p, ctx->texcoords.p,
                                                    ctx->shader_ids.p);
-  k_node_records<<<blocks(n, 256), 256, 0, s>>>(ctx->vertices.p, n, ctx->nodes.p);
+  k_node_records<<<blocks((uint64_t)n + 1, 256), 256, 0, s>>>(ctx->vertices.p, n, ctx->nodes.p);
   AUR_LAUNCH_CHECK(ctx, "gather/node records");
   AUR_HIP(ctx, hipEventRecord(e1, s));
   AUR_HIP(ctx, hipEventSynchronize(e1));

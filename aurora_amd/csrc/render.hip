@@ -29,7 +29,7 @@ namespace aur {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kRefill = 16;  // persistent traversal: refill once 16 lanes idle
+constexpr uint32_t kRefill = 16;  // persistent traversal: refill once 16 lanes idle
 
 struct SceneDev {
   const NodeRec* nodes;
@@ -453,8 +453,8 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
     a.counters[cin ^ 1] = 0;
     atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
-  persistent_trace<false, kRefill>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+  persistent_trace<false>(
+      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, kRefill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t j, V3& o, V3& d, float& tmax, TreeRef& tree) {
         const uint32_t p = queue[j];
         if constexpr (kMulti) {
@@ -611,8 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
   const uint32_t n_in = a.counters[cin];
   const uint64_t total = (uint64_t)n_in * a.nslots;
   uint32_t traced = 0;
-  persistent_trace<true, kRefill>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, total, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+  persistent_trace<true>(
+      TreeRef{a.job.nodes, a.job.n_nodes}, total, kRefill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree) {
         const uint32_t sl = (uint32_t)(i / n_in), j = (uint32_t)(i - (uint64_t)sl * n_in);
         const size_t e = (size_t)sl * a.cap + j;

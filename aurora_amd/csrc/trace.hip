@@ -1,8 +1,8 @@
 // Standalone traversal launches (src/kernels/lib/intersect.cuh:36-152).
 //
-// k_trace_pt: persistent waves (persistent_trace in traverse.h): each wave
-// owns a contiguous range of rays and refills idle lanes from it, so lanes
-// whose rays finish early do not idle until the slowest ray of the wave ends.
+// k_trace_pt: persistent waves (trace_rays below): each wave owns a
+// contiguous range of rays and refills idle lanes from it, so lanes whose
+// rays finish early do not idle until the slowest ray of the wave ends.
 // Each lane keeps its deferred ranges in an LDS column of levels - 1 float2
 // entries (or the first rows in LDS and the rest in global memory).
 //
@@ -28,32 +28,71 @@ __device__ __forceinline__ void store_result(uint64_t i, const Trav& s, aur_hit*
     reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(s.id), s.t, s.u, s.v);
 }
 
-template <bool kAny, int kRefill, bool kSplit>
+// Persistent traversal of rays [0, n) with the refill of persistent_trace;
+// the tree is wave-uniform, so its root slab is read once per wave and a
+// refill waits on the ray loads only.  (Keeping the next 64 rays in
+// registers and handing them out by lane permutes measured slower: the
+// permutes' register reuse makes every refill wait for them.)
+template <bool kAny, class Stk>
+__device__ __forceinline__ void trace_rays(TreeRef tree, const aur_ray* __restrict__ rays, uint64_t n,
+                                           uint32_t refill, const Stk& stk,
+                                           aur_hit* __restrict__ hits, int32_t* __restrict__ ids) {
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t chunk = (n + waves - 1) / waves;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  uint64_t next = wave * chunk;
+  const uint64_t end = next + chunk < n ? next + chunk : n;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const float2 root = root_slab(tree.nodes, tree.n_nodes);
+  Trav s;
+  bool active = false;
+  uint64_t item = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (nidle >= refill || nidle == 64) {
+      if (next < end) {
+        if (!active) {
+          const uint64_t i = next + __popcll(idle & lt);
+          if (i < end) {
+            const float4* r = reinterpret_cast<const float4*>(rays + i);
+            const float4 a = r[0], b = r[1];
+            item = i;
+            active = trav_init_root<false>(s, root, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                                           nullptr);
+            if (!active) store_result(i, s, hits, ids);
+          }
+        }
+        next += nidle;
+      } else if (nidle == 64) {
+        break;
+      }
+    }
+    if (active && trav_step<kAny, false>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
+      store_result(item, s, hits, ids);
+      active = false;
+    }
+  }
+}
+
+template <bool kAny, bool kSplit>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restrict__ nodes,
                                                         uint32_t n_nodes,
                                                         const aur_ray* __restrict__ rays, uint64_t n,
                                                         aur_hit* __restrict__ hits,
-                                                        int32_t* __restrict__ ids,
+                                                        int32_t* __restrict__ ids, uint32_t refill,
                                                         float2* __restrict__ gstack, uint32_t klds) {
   extern __shared__ float2 stk_mem[];
-  const auto fetch = [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef&) {
-    const float4* r = reinterpret_cast<const float4*>(rays + i);
-    const float4 a = r[0], b = r[1];
-    o = mk(a.x, a.y, a.z);
-    d = mk(b.x, b.y, b.z);
-    tmax = a.w;
-    return true;
-  };
-  const auto store = [&](uint64_t i, const Trav& st) { store_result(i, st, hits, ids); };
   if constexpr (kSplit) {
     // the first klds stack rows in LDS, deeper ones in global memory
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const SplitStack stk{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
                          (uint64_t)gridDim.x * blockDim.x, klds};
-    persistent_trace<kAny, kRefill>(TreeRef{nodes, n_nodes}, n, stk, fetch, store);
+    trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill, stk, hits, ids);
   } else {
-    persistent_trace<kAny, kRefill>(TreeRef{nodes, n_nodes}, n,
-                                    lds_stack(stk_mem + threadIdx.x, blockDim.x), fetch, store);
+    trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill,
+                     lds_stack(stk_mem + threadIdx.x, blockDim.x), hits, ids);
   }
 }
 
@@ -69,7 +108,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restr
   const float4* r = reinterpret_cast<const float4*>(rays + i);
   const float4 a = r[0], b = r[1];
   Trav s;
-  if (trav_init<false>(s, nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
+  if (trav_init<false>(s, nodes, n_nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
     const LdsStack stk = lds_stack(stk_mem + threadIdx.x, blockDim.x);
     while (!trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
     }
@@ -110,7 +149,7 @@ __global__ void k_zero2(unsigned long long* p) {
 
 // Self-check of the division shortcuts used by the traversal:
 //   sums[0] += mismatches of recip_rn(x) against 1.0f / x over every float x
-//              in [2^-24, 2^64] (bit patterns lo..hi);
+//              with |x| in [2^-60, 2^64] (bit patterns lo..hi, both signs);
 //   sums[1] += mismatches of quot_fast(n, d, RN(1/d)) against n / d over
 //              `samples` hashed pairs with 2^-60 <= |d| < 16 and n = 0 or
 //              2^-96 <= |n| < 2^60, random signs and mantissas.
@@ -130,6 +169,7 @@ __global__ __launch_bounds__(256) void k_division_check(uint32_t lo, uint32_t hi
   for (uint64_t b = lo + gtid; b <= hi; b += stride) {
     const float x = __uint_as_float((uint32_t)b);
     bad_r += __float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x);
+    bad_r += __float_as_uint(recip_rn(-x)) != __float_as_uint(1.0f / -x);
   }
   for (uint64_t i = gtid; i < samples; i += stride) {
     const uint64_t h = mix64(seed ^ mix64(i));
@@ -175,17 +215,11 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   const size_t lds_pt = (size_t)rows * block * sizeof(float2);
   // resident waves: occupancy-limited blocks per CU times CUs (or the request)
   int per_cu = 0;
+  if (tp.refill < 1 || tp.refill > 64)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be in 1..64");
   using Kern = void (*)(const NodeRec*, uint32_t, const aur_ray*, uint64_t, aur_hit*, int32_t*,
-                        float2*, uint32_t);
-  Kern fn = nullptr;
-  const bool split = klds != 0;
-  switch (tp.refill) {
-    case 1: fn = split ? k_trace_pt<kAny, 1, true> : k_trace_pt<kAny, 1, false>; break;
-    case 8: fn = split ? k_trace_pt<kAny, 8, true> : k_trace_pt<kAny, 8, false>; break;
-    case 16: fn = split ? k_trace_pt<kAny, 16, true> : k_trace_pt<kAny, 16, false>; break;
-    case 32: fn = split ? k_trace_pt<kAny, 32, true> : k_trace_pt<kAny, 32, false>; break;
-    default: return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be 1, 8, 16 or 32");
-  }
+                        uint32_t, float2*, uint32_t);
+  const Kern fn = klds != 0 ? k_trace_pt<kAny, true> : k_trace_pt<kAny, false>;
   AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, (int)block, lds_pt));
   if (per_cu < 1) per_cu = 1;
   uint64_t blocks_total = (uint64_t)per_cu * ctx->num_cus;
@@ -200,7 +234,7 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     gstack = ctx->gstack.p;
   }
   hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds_pt, ctx->stream, ctx->nodes.p, ctx->count / 2,
-                     rays, n, hits, ids, gstack, klds);
+                     rays, n, hits, ids, tp.refill, gstack, klds);
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;
 }
@@ -236,13 +270,13 @@ aur_status division_check(aur_context* ctx, uint64_t samples, uint64_t seed, uin
   DevBuf<unsigned long long> sums;
   AUR_HIP(ctx, sums.reserve(2));
   k_zero2<<<1, 64, 0, ctx->stream>>>(sums.p);
-  const uint32_t lo = 0x33800000u, hi = 0x5F800000u;  // 2^-24 .. 2^64
+  const uint32_t lo = 0x21800000u, hi = 0x5F800000u;  // 2^-60 .. 2^64
   k_division_check<<<4096, 256, 0, ctx->stream>>>(lo, hi, samples, seed, sums.p);
   AUR_LAUNCH_CHECK(ctx, "k_division_check");
   unsigned long long h[2] = {0, 0};
   AUR_HIP(ctx, hipMemcpyAsync(h, sums.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (recip_checked) *recip_checked = (uint64_t)(hi - lo) + 1;
+  if (recip_checked) *recip_checked = 2 * ((uint64_t)(hi - lo) + 1);
   if (recip_bad) *recip_bad = h[0];
   if (quot_bad) *quot_bad = h[1];
   return AUR_OK;

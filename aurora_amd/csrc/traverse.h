@@ -22,13 +22,16 @@
 //         q0 = RN(n y),  e = n - d q0 (exact, FMA),  q = RN(q0 + e y),
 //     which equals RN(n / d) (Markstein's theorem) while no step under- or
 //     overflows: used only for 2^-60 <= |d| <= 16 and n = 0 or
-//     2^-96 <= |n| <= 2^60, true division otherwise;
+//     2^-96 <= |n| <= 2^60, true division otherwise.  Inside the loop the
+//     numerator range follows from a per-node flag (all four child slab
+//     values pass normal_coord) and a per-ray check of the origin;
 //   * both triangles of a node are tested without branches (every value is
 //     computed, the reference's early outs become one accept predicate with
 //     the same NaN behaviour), and 1/det is HIP's correctly rounded
 //     reciprocal sequence without its scaling and fix-up steps, which are
 //     identities for 2^-24 <= det <= 2^64 (det < kEpsilon is rejected before
-//     the value is used; det > 2^64 takes true division).
+//     the value is used; det > 2^64 takes true division); the ray's
+//     reciprocals use the same sequence (needed only for 2^-60 <= |d| <= 16).
 //
 // The traversal is a small state machine (trav_init / trav_step, one node
 // visit per step) so that persistent kernels can refill finished lanes.
@@ -66,15 +69,15 @@ __device__ __forceinline__ float2 narrow(float2 r, float slo, float shi, float p
   return clip(r, d0, d1);
 }
 
-// Both children's slabs on one axis: one shared (rarely taken) branch to
-// true division.
+// Both children's slabs on one axis; `fast`: the divisor and all four
+// numerators are in the reciprocal quotient's range (else one shared, rarely
+// taken branch to true division).
 __device__ __forceinline__ void narrow2(float2 r, float4 slabs, float p, float d, float y,
-                                        bool fast_d, float2& rl, float2& rr) {
+                                        bool fast, float2& rl, float2& rr) {
   const float n0 = slabs.x - p, n1 = slabs.y - p, n2 = slabs.z - p, n3 = slabs.w - p;
   float d0 = quot_fast(n0, d, y), d1 = quot_fast(n1, d, y);
   float d2 = quot_fast(n2, d, y), d3 = quot_fast(n3, d, y);
-  const bool f0 = fast_num(n0), f1 = fast_num(n1), f2 = fast_num(n2), f3 = fast_num(n3);
-  if (!(fast_d & f0 & f1 & f2 & f3)) {
+  if (!fast) {
     d0 = n0 / d;
     d1 = n1 / d;
     d2 = n2 / d;
@@ -84,7 +87,7 @@ __device__ __forceinline__ void narrow2(float2 r, float4 slabs, float p, float d
   rr = clip(r, d2, d3);
 }
 
-// RN(1/x) for 2^-24 <= x <= 2^64: the steps of HIP's IEEE division 1/x
+// RN(1/x) for 2^-60 <= |x| <= 2^64: the steps of HIP's IEEE division 1/x
 // (reciprocal estimate, one Newton step, quotient and two residual
 // corrections) without the operand scaling and result fix-up, which do
 // nothing on this range.
@@ -123,7 +126,7 @@ struct TraceCount {
 
 struct Trav {
   V3 o, d, y;      // origin, direction, RN(1/direction)
-  uint32_t fast;   // bit a: axis a uses the reciprocal quotient
+  uint32_t fast;   // bit a: 2^-60 <= |d_a| <= 16 and normal_coord(o_a)
   float t, u, v;   // current closest hit (t = ray.t)
   int32_t id;
   uint32_t k, depth, trail;
@@ -171,28 +174,44 @@ __device__ __forceinline__ LdsStack lds_stack(float2* lane_column, uint32_t stri
   return LdsStack{lane_column, stride};
 }
 
+__device__ __forceinline__ uint32_t fast_axis(float o, float d) {
+  const float a = fabsf(d);
+  return (uint32_t)((a >= 0x1p-60f) & (a <= 16.0f) & normal_coord(o));
+}
+
+// The root's own slab: cslab[0..1] of the record after the last node.
+__device__ __forceinline__ float2 root_slab(const NodeRec* __restrict__ nodes, uint32_t n_nodes) {
+  const float4 rs = reinterpret_cast<const float4*>(nodes + n_nodes)[4];
+  return make_float2(rs.z, rs.w);
+}
+
 // Sets up a ray; returns false when the root's slab culls it (finished, miss).
+// The reciprocals y serve only axes with 2^-60 <= |d| <= 16, where recip_rn
+// equals 1.0f / d.
 template <bool kCount>
-__device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ nodes, V3 o, V3 d,
-                                          float tmax, TraceCount* cnt) {
+__device__ __forceinline__ bool trav_init_root(Trav& s, float2 root, V3 o, V3 d, float tmax,
+                                               TraceCount* cnt) {
   s.o = o;
   s.d = d;
-  s.y = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  s.y = mk(recip_rn(d.x), recip_rn(d.y), recip_rn(d.z));
   // keep the reciprocals in registers: without this barrier the compiler
-  // re-derives 1/d (a full division) at every use inside the loop
+  // re-derives them at every use inside the loop
   __asm__ volatile("" : "+v"(s.y.x), "+v"(s.y.y), "+v"(s.y.z));
-  s.fast = 0;
-  if (fabsf(d.x) >= 0x1p-60f && fabsf(d.x) <= 16.0f) s.fast |= 1u;
-  if (fabsf(d.y) >= 0x1p-60f && fabsf(d.y) <= 16.0f) s.fast |= 2u;
-  if (fabsf(d.z) >= 0x1p-60f && fabsf(d.z) <= 16.0f) s.fast |= 4u;
+  s.fast = fast_axis(o.x, d.x) | (fast_axis(o.y, d.y) << 1) | (fast_axis(o.z, d.z) << 2);
   s.t = tmax;
   s.u = s.v = 0.0f;
   s.id = -1;
   s.k = s.depth = s.trail = s.axis = 0;
   if (kCount) cnt->popped += 1;  // the root
-  const float4 rs = reinterpret_cast<const float4*>(nodes)[5];  // node 0: cslab.zw, own slab
-  s.r = narrow(make_float2(kEpsilon, kInfinity), rs.z, rs.w, o.x, d.x, s.y.x, s.fast & 1u);
+  s.r = narrow(make_float2(kEpsilon, kInfinity), root.x, root.y, o.x, d.x, s.y.x, s.fast & 1u);
   return (s.r.x <= s.r.y) && (s.r.x <= s.t);
+}
+
+template <bool kCount>
+__device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ nodes,
+                                          uint32_t n_nodes, V3 o, V3 d, float tmax,
+                                          TraceCount* cnt) {
+  return trav_init_root<kCount>(s, root_slab(nodes, n_nodes), o, d, tmax, cnt);
 }
 
 // Visits node s.k; returns true when the traversal has finished.  All paths
@@ -213,7 +232,7 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   // would wait a second memory latency per visit)
   const float4* rec = reinterpret_cast<const float4*>(nodes + k);
   const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
-  const float2 cs = reinterpret_cast<const float2*>(rec + 5)[0];
+  const float4 a5 = rec[5];  // right child's slab, flags
   // both triangles up to their determinants, then both reciprocals
   const V3 v1a = mk(a0.x, a0.y, a0.z), e1a = mk(a0.w, a1.x, a1.y), e2a = mk(a1.z, a1.w, a2.x);
   const V3 v1b = mk(a2.y, a2.z, a2.w), e1b = mk(a3.x, a3.y, a3.z), e2b = mk(a3.w, a4.x, a4.y);
@@ -250,9 +269,9 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     const uint32_t cd = s.depth + 1;
     const int ca = s.axis == 2 ? 0 : (int)s.axis + 1;
     const float oc = compf(s.o, ca), dc = compf(s.d, ca), yc = compf(s.y, ca);
-    const bool fd = (s.fast >> ca) & 1u;
+    const bool fast = ((s.fast >> ca) & __float_as_uint(a5.z) & kNodeSlabsNormal) != 0u;
     float2 rl, rr;
-    narrow2(s.r, make_float4(a4.z, a4.w, cs.x, cs.y), oc, dc, yc, fd, rl, rr);
+    narrow2(s.r, make_float4(a4.z, a4.w, a5.x, a5.y), oc, dc, yc, fast, rl, rr);
     const bool okl = rl.x <= rl.y;
     const bool okr = (cl + 1 < n_nodes) & (rr.x <= rr.y);
     const bool sw = dc < 0.0f;
@@ -273,7 +292,12 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   if (!stop && !found) {
     while (s.trail) {
       const uint32_t dd = 31u - (uint32_t)__clz(s.trail);
-      const float2 fr = stk.load((uint32_t)__popc(s.trail) - 1u);
+      const float2 frv = stk.load((uint32_t)__popc(s.trail) - 1u);
+      // both halves in one LDS read (else the second is issued after the
+      // test and the next step's loads wait for it)
+      float fx = frv.x, fy = frv.y;
+      __asm__ volatile("" : "+v"(fx), "+v"(fy));
+      const float2 fr = make_float2(fx, fy);
       s.trail &= ~(1u << dd);
       if (fr.x <= s.t) {
         const uint32_t anc = ((k + 1) >> (s.depth - dd)) - 1;
@@ -300,7 +324,7 @@ struct TreeRef {
 };
 
 // Persistent-wave traversal of a work range [0, n).  Wave w owns
-// [w*chunk, (w+1)*chunk); between node visits, once at least kRefill lanes
+// [w*chunk, (w+1)*chunk); between node visits, once at least `refill` lanes
 // are idle (or all are), idle lanes take the next items of the range, ranked
 // by ballot (no atomics).  fetch(i, o, d, tmax, tree) loads item i, may
 // replace the lane's tree (initially tree0; batches of scenes), and returns
@@ -308,9 +332,9 @@ struct TreeRef {
 // state.  A fetch that never writes `tree` keeps the tree wave-uniform.
 // (Splitting a refill over two iterations, so that the item loads overlap
 // the active lanes' node step, measured slower: the new lanes lose a step.)
-template <bool kAny, int kRefill, class Stk, class Fetch, class Store>
-__device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, const Stk& stk,
-                                                 Fetch fetch, Store store) {
+template <bool kAny, class Stk, class Fetch, class Store>
+__device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint32_t refill,
+                                                 const Stk& stk, Fetch fetch, Store store) {
   TreeRef tree = tree0;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t chunk = (n + waves - 1) / waves;
@@ -325,7 +349,7 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, cons
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (nidle >= (uint32_t)kRefill || nidle == 64) {
+    if (nidle >= refill || nidle == 64) {
       if (next < end) {
         if (!active) {
           const uint64_t i = next + __popcll(idle & lt);
@@ -334,7 +358,7 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, cons
             float tmax;
             if (fetch(i, o, d, tmax, tree)) {
               item = i;
-              active = trav_init<false>(s, tree.nodes, o, d, tmax, nullptr);
+              active = trav_init<false>(s, tree.nodes, tree.n_nodes, o, d, tmax, nullptr);
               if (!active) store(i, s);
             }
           }
@@ -359,7 +383,7 @@ __device__ __forceinline__ TraceResult trace_ray(const NodeRec* __restrict__ nod
                                                  uint32_t stride, TraceCount* cnt = nullptr) {
   Trav s;
   const LdsStack st = lds_stack(stk, stride);
-  if (trav_init<kCount>(s, nodes, o, d, tmax, cnt)) {
+  if (trav_init<kCount>(s, nodes, n_nodes, o, d, tmax, cnt)) {
     while (!trav_step<kAny, kCount>(s, nodes, n_nodes, st, cnt)) {
     }
   }

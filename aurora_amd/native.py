@@ -275,7 +275,7 @@ class Context:
         self._check(self._lib.aur_trace_closest(self._h, n, ctypes.c_void_p(rays_ptr),
                                                 ctypes.c_void_p(hits_ptr)), "trace_closest")
 
-    def set_trace_params(self, persistent: int = 1, block: int = 256, refill: int = 16,
+    def set_trace_params(self, persistent: int = 1, block: int = 256, refill: int = 24,
                          waves_per_cu: int = 0, stack_lds_rows: int = 0) -> None:
         p = TraceParams(persistent, block, refill, waves_per_cu, stack_lds_rows)
         self._check(self._lib.aur_set_trace_params(self._h, ctypes.byref(p)), "set_trace_params")

@@ -24,8 +24,9 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "t
 
 TRACE_CANDIDATES = (
     [dict(persistent=0, block=b, refill=1, waves_per_cu=0, stack_lds_rows=0) for b in (64, 256)]
-    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0, stack_lds_rows=s) for b in (128, 256)
-       for r in (8, 16, 32) for s in (0, 10, 14)]
+    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0, stack_lds_rows=0) for b in (128, 256)
+       for r in (16, 24, 32, 40, 48)]
+    + [dict(persistent=1, block=256, refill=32, waves_per_cu=0, stack_lds_rows=s) for s in (10, 14)]
 )
 
 

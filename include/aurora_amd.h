@@ -160,7 +160,7 @@ aur_status aur_trace_any(aur_context* ctx, uint64_t n, const aur_ray* rays, int3
  * reference launched one thread per ray in blocks of 64,
  * src/kernels/monteCarloRaytracer.cu:119-124).  persistent = 1: resident
  * waves each own a contiguous ray range and refill idle lanes once `refill`
- * (1, 8, 16 or 32) lanes are idle; waves_per_cu = 0 uses the occupancy
+ * (1..64) lanes are idle; waves_per_cu = 0 uses the occupancy
  * limit.  persistent = 0: one lane per ray.  block: 64, 128 or 256.
  * stack_lds_rows: how many rows of each lane's stack of deferred ranges
  * (a compact stack of at most levels - 1 entries) live in LDS (0 = all);
@@ -178,9 +178,9 @@ aur_status aur_get_trace_params(aur_context* ctx, aur_trace_params* params);
 aur_status aur_trace_closest_stats(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
                                    uint64_t* popped, uint64_t* passed);
 /* Device self-check of the traversal's division shortcuts (no reference
- * counterpart: the reference divides directly).  recip_bad counts floats x in
- * [2^-24, 2^64] (all recip_checked of them) whose shortcut reciprocal differs
- * from 1.0f / x; quot_bad counts, over `samples` hashed pairs (n, d) in the
+ * counterpart: the reference divides directly).  recip_bad counts floats x
+ * with |x| in [2^-60, 2^64] (all recip_checked of them) whose shortcut
+ * reciprocal differs from 1.0f / x; quot_bad counts, over `samples` hashed pairs (n, d) in the
  * shortcut's range, reciprocal quotients that differ from n / d.  Both must be
  * 0.  Synchronous. */
 aur_status aur_check_division(aur_context* ctx, uint64_t samples, uint64_t seed,

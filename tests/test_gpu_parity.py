@@ -125,11 +125,40 @@ def test_trace_soup_with_ties(gpu_ctx, oracle):
     assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32))
 
 
+def test_trace_launch_shapes_agree(gpu_ctx, oracle):
+    """Every launch shape (one lane per ray, persistent waves with any refill
+    threshold, stack partly in global memory) gives the oracle's hits."""
+    s = scenes.spheres(n_spheres=40)
+    rays = scenes.incoherent_rays(100_000, 0.0, 100.0, seed=17)
+    shadow = rays.copy()
+    shadow[:, 3] = np.random.default_rng(4).uniform(1.0, 60.0, rays.shape[0]).astype(np.float32)
+    perm, _ = _build(gpu_ctx, s)
+    ids_o, tuv_o = oracle.trace(s.vertices[perm], rays)
+    any_o, _ = oracle.trace(s.vertices[perm], shadow, any_hit=True)
+    shapes = [dict(persistent=0, block=64, refill=1), dict(persistent=1, block=128, refill=1),
+              dict(persistent=1, block=256, refill=7), dict(persistent=1, block=256, refill=64),
+              dict(persistent=1, block=256, refill=32, stack_lds_rows=3),
+              dict(persistent=1, block=128, refill=24, stack_lds_rows=12)]
+    try:
+        for p in shapes:
+            gpu_ctx.set_trace_params(**p)
+            ids_g, tuv_g = gpu_ctx.trace(rays)
+            assert np.array_equal(ids_g, ids_o), p
+            assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32)), p
+            any_g, _ = gpu_ctx.trace(shadow, any_hit=True)
+            assert np.array_equal(any_g, any_o), p
+        with pytest.raises(native.AuroraError):
+            gpu_ctx.set_trace_params(persistent=1, block=256, refill=65)
+    finally:
+        gpu_ctx.set_trace_params()
+
+
 def test_division_shortcuts_exact(gpu_ctx):
-    """The traversal's reciprocal shortcut equals 1.0f/x for every float in
-    [2^-24, 2^64]; its slab quotient equals n/d on 2^30 sampled pairs."""
+    """The traversal's reciprocal shortcut equals 1.0f/x for every float x
+    with |x| in [2^-60, 2^64]; its slab quotient equals n/d on 2^30 sampled
+    pairs."""
     checked, recip_bad, quot_bad = gpu_ctx.check_division(1 << 30, seed=5)
-    assert checked == 0x5F800000 - 0x33800000 + 1
+    assert checked == 2 * (0x5F800000 - 0x21800000 + 1)
     assert recip_bad == 0
     assert quot_bad == 0
 
