@@ -82,13 +82,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                                                         const aur_ray* __restrict__ rays, uint64_t n,
                                                         aur_hit* __restrict__ hits,
                                                         int32_t* __restrict__ ids, uint32_t refill,
-                                                        float2* __restrict__ gstack, uint32_t klds) {
+                                                        float2* __restrict__ gstack, uint32_t klds,
+                                                        uint32_t gstack_rows) {
   extern __shared__ float2 stk_mem[];
   if constexpr (kSplit) {
-    // the first klds stack rows in LDS, deeper ones in global memory
-    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const SplitStack stk{stk_mem + threadIdx.x, blockDim.x, gstack + gtid,
-                         (uint64_t)gridDim.x * blockDim.x, klds};
+    // the first klds stack rows in LDS, deeper ones in global memory (the
+    // launch keeps the buffer under 2^32 bytes)
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t bytes = (gstack_rows * lanes) * 8u;
+    const SplitStack stk{(lds_vf2*)(stk_mem + threadIdx.x), blockDim.x,
+                         (glb_vf2*)(gstack + gtid),
+                         __builtin_amdgcn_make_buffer_rsrc(gstack, (short)0, (int)bytes, 0x00020000),
+                         gtid * 8u, lanes, klds};
     trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill, stk, hits, ids);
   } else {
     trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill,
@@ -218,7 +224,7 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   if (tp.refill < 1 || tp.refill > 64)
     return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be in 1..64");
   using Kern = void (*)(const NodeRec*, uint32_t, const aur_ray*, uint64_t, aur_hit*, int32_t*,
-                        uint32_t, float2*, uint32_t);
+                        uint32_t, float2*, uint32_t, uint32_t);
   const Kern fn = klds != 0 ? k_trace_pt<kAny, true> : k_trace_pt<kAny, false>;
   AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, (int)block, lds_pt));
   if (per_cu < 1) per_cu = 1;
@@ -230,11 +236,14 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   const uint32_t grid = (uint32_t)((nwaves * 64 + block - 1) / block);
   float2* gstack = nullptr;
   if (klds) {
-    AUR_HIP(ctx, ctx->gstack.reserve((size_t)(maxrows - rows) * grid * block));
+    const size_t entries = (size_t)(maxrows - rows) * grid * block;
+    if (entries * sizeof(float2) >= (1ull << 31))
+      return ctx->fail(AUR_ERR_INVALID_ARG, "trace: global stack rows exceed 2 GiB");
+    AUR_HIP(ctx, ctx->gstack.reserve(entries));
     gstack = ctx->gstack.p;
   }
   hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds_pt, ctx->stream, ctx->nodes.p, ctx->count / 2,
-                     rays, n, hits, ids, tp.refill, gstack, klds);
+                     rays, n, hits, ids, tp.refill, gstack, klds, maxrows - rows);
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;
 }

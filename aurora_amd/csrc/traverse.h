@@ -152,21 +152,36 @@ struct LdsStack {
 
 // SplitStack: rows [0, klds) in LDS, deeper rows in a per-lane column of
 // global memory, so that a small LDS window leaves room for more resident
-// waves.  (Its accesses go through generic pointers.)
+// waves.  The LDS side uses an LDS-qualified pointer and the global side
+// stores through a global pointer and loads through a buffer resource, so
+// the two stay separate instructions (merged, they would become generic
+// "flat" accesses, slower for the common LDS case).
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) vf2 lds_vf2;
+typedef __attribute__((address_space(1))) vf2 glb_vf2;
 struct SplitStack {
-  float2* lds;
+  lds_vf2* lds;            // row r < klds at lds[r * lstride]
   uint32_t lstride;
-  float2* glb;  // row r >= klds at glb[(r - klds) * gstride]
-  uint64_t gstride;
+  glb_vf2* glb;            // row r >= klds at glb[(r - klds) * gstride]
+  __amdgpu_buffer_rsrc_t rsrc;  // the whole global stack buffer
+  uint32_t lane_bytes;     // byte offset of glb from the buffer start
+  uint32_t gstride;
   uint32_t klds;
   __device__ __forceinline__ void store(uint32_t r, float2 v) const {
+    const vf2 w = {v.x, v.y};
     if (r < klds)
-      lds[r * lstride] = v;
+      lds[r * lstride] = w;
     else
-      glb[(r - klds) * gstride] = v;
+      glb[(size_t)(r - klds) * gstride] = w;
   }
   __device__ __forceinline__ float2 load(uint32_t r) const {
-    return r < klds ? lds[r * lstride] : glb[(r - klds) * gstride];
+    if (r < klds) {
+      const vf2 w = lds[r * lstride];
+      return make_float2(w.x, w.y);
+    }
+    const auto w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane_bytes + (r - klds) * gstride * 8u,
+                                                        0, 0);
+    return make_float2(__uint_as_float(w[0]), __uint_as_float(w[1]));
   }
 };
 
