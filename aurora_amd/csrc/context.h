@@ -51,7 +51,8 @@ struct DevBuf {
 
 // Default paths in flight per PBR wavefront pass (2^21 paths: about 0.2 GB of
 // path state plus 48 B per next-event slot and path).
-constexpr uint32_t kDefaultMaxPaths = 1u << 21;
+constexpr uint32_t kDefaultMaxPaths = 1u << 24;
+constexpr uint32_t kDefaultRenderRefill = 16;
 
 // One render job of the PBR wavefront: a camera over one built scene.  A
 // frame render is a single job; a batch is many.  Paths are numbered
@@ -99,6 +100,7 @@ struct FrameState {
   DevBuf<uint32_t> counters;  // [0..1] queue lengths, [2] shadow count, [3..] stats
   DevBuf<float4> sh_o, sh_d, sh_c;  // shadow rays: origin+tmax, dir, contribution
   DevBuf<int32_t> sh_occ;
+  DevBuf<uint32_t> sh_queue;  // slots holding a shadow ray, appended by shading
   // adjoint replay: path-vertex records, per-block partial gradients,
   // running gradient sums and the largest |dLoss/dI|
   DevBuf<int32_t> rec_mat;
@@ -125,7 +127,7 @@ struct aur_context {
   // limited residency: fastest shape measured at 1M triangles / 2^24 rays
   aur_trace_params trace_params{1, 256, 24, 0, 0};
   aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
-  aur_render_params render_params{aur::kDefaultMaxPaths, 0};
+  aur_render_params render_params{aur::kDefaultMaxPaths, 0, aur::kDefaultRenderRefill};
 
   // uploaded geometry (input order)
   uint32_t count = 0;

@@ -29,7 +29,6 @@ namespace aur {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr uint32_t kRefill = 16;  // persistent traversal: refill once 16 lanes idle
 
 struct SceneDev {
   const NodeRec* nodes;
@@ -342,11 +341,13 @@ struct PbrArgs {
   const unsigned long long* g0p;
   unsigned long long end;
   uint32_t cap, nslots, max_depth;
+  uint32_t refill;  // persistent traversal: refill once this many lanes are idle
   float4 *ray_o, *ray_d, *beta, *radiance;
   aur_hit* hits;
   float4 *sh_o, *sh_d, *sh_c;
   int32_t* sh_occ;
-  uint32_t* counters;  // [0],[1] queue lengths
+  uint32_t* sh_queue;  // slot indices e = slot * cap + queue position with a shadow ray
+  uint32_t* counters;  // [0],[1] path queue lengths, [2] shadow queue length
   unsigned long long* stats;
   // path-vertex records of an adjoint replay (null in a plain render),
   // indexed [bounce][q] and [bounce][slot][q] with stride cap: shader index of
@@ -448,13 +449,15 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
   extern __shared__ float2 stk_mem[];
   const uint32_t n_in = a.counters[cin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the other queue's length was last read by the previous bounce's
-    // kernels, which have completed; the shade kernel that follows appends
+    // the other queue's length and the shadow queue's were last read by the
+    // previous bounce's kernels, which have completed; the shade kernel that
+    // follows appends to both
     a.counters[cin ^ 1] = 0;
+    a.counters[2] = 0;
     atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
   persistent_trace<false>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, kRefill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, a.refill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t j, V3& o, V3& d, float& tmax, TreeRef& tree) {
         const uint32_t p = queue[j];
         if constexpr (kMulti) {
@@ -468,6 +471,20 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
         return true;
       },
       [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
+}
+
+// Appends slot index e to the shadow queue: one atomic per wave for the lanes
+// appending together (a single-job pass visits lights in the same order in
+// every lane, so a wave's items stay grouped by slot).
+__device__ __forceinline__ void append_shadow(const PbrArgs& a, uint32_t e) {
+  const uint64_t m = __ballot(true);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+  const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t basepos = 0;
+  if (lane == leader) basepos = atomicAdd(&a.counters[2], (uint32_t)__popcll(m));
+  basepos = __shfl(basepos, leader);
+  a.sh_queue[basepos + rank] = e;
 }
 
 template <bool kMulti, bool kRecord>
@@ -530,6 +547,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
           a.sh_o[base + e * ss] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
           a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, wgt * kInvPi);
           flag = 1.0f;
+          append_shadow(a, (uint32_t)(e * ss + base));
         }
         a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       }
@@ -544,6 +562,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
         a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, dist);
         a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS / (dist * dist) * kInvPi);
         flag = 1.0f;
+        append_shadow(a, (uint32_t)(e * ss + base));
       }
       a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
@@ -556,6 +575,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
         a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, kInfinity);
         a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS * kInvPi);
         flag = 1.0f;
+        append_shadow(a, (uint32_t)(e * ss + base));
       }
       a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
@@ -602,38 +622,28 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   qout[basepos + rank] = p;
 }
 
-// Any hit for every next-event slot that needs a shadow ray.
+// Any hit for every queued shadow ray (the slots shading filled).
 template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
                                                        const uint32_t* __restrict__ qin) {
   extern __shared__ float2 stk_mem[];
-  // items slot-major: one wave traces neighbouring paths' rays to the same light
-  const uint32_t n_in = a.counters[cin];
-  const uint64_t total = (uint64_t)n_in * a.nslots;
-  uint32_t traced = 0;
+  const uint32_t total = a.counters[2];
   persistent_trace<true>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, total, kRefill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
+      TreeRef{a.job.nodes, a.job.n_nodes}, total, a.refill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree) {
-        const uint32_t sl = (uint32_t)(i / n_in), j = (uint32_t)(i - (uint64_t)sl * n_in);
-        const size_t e = (size_t)sl * a.cap + j;
-        const float4 d4 = a.sh_d[e];
-        if (d4.w != 1.0f) return false;
+        const uint32_t e = a.sh_queue[i];
         if constexpr (kMulti) {
-          const JobDev& J = a.jobs[a.path_job[qin[j]]];
+          const JobDev& J = a.jobs[a.path_job[qin[e % a.cap]]];
           tree = TreeRef{J.nodes, J.n_nodes};
         }
-        const float4 o4 = a.sh_o[e];
+        const float4 o4 = a.sh_o[e], d4 = a.sh_d[e];
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
         tmax = o4.w;
-        ++traced;
         return true;
       },
-      [&](uint64_t i, const Trav& s) {
-        const uint32_t sl = (uint32_t)(i / n_in), j = (uint32_t)(i - (uint64_t)sl * n_in);
-        a.sh_occ[(size_t)sl * a.cap + j] = s.id >= 0 ? 1 : 0;
-      });
-  count_add(&a.stats[1], traced);
+      [&](uint64_t i, const Trav& s) { a.sh_occ[a.sh_queue[i]] = s.id >= 0 ? 1 : 0; });
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[1], (unsigned long long)total);
 }
 
 template <bool kRecord>
@@ -896,6 +906,7 @@ aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t
   AUR_HIP(ctx, fr.sh_d.reserve(nsh));
   AUR_HIP(ctx, fr.sh_c.reserve(nsh));
   AUR_HIP(ctx, fr.sh_occ.reserve(nsh));
+  AUR_HIP(ctx, fr.sh_queue.reserve(nsh));
   if (records) {
     AUR_HIP(ctx, fr.rec_mat.reserve((size_t)cap * max_depth));
     AUR_HIP(ctx, fr.rec_rr.reserve((size_t)cap * max_depth));
@@ -970,6 +981,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   a.cap = cap;
   a.nslots = run.nslots;
   a.max_depth = run.max_depth;
+  a.refill = ctx->render_params.refill;
   a.ray_o = fr.ray_o.p;
   a.ray_d = fr.ray_d.p;
   a.beta = fr.beta.p;
@@ -979,6 +991,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   a.sh_d = fr.sh_d.p;
   a.sh_c = fr.sh_c.p;
   a.sh_occ = fr.sh_occ.p;
+  a.sh_queue = fr.sh_queue.p;
   a.counters = fr.counters.p;
   a.stats = stats;
   if (adj) {

@@ -98,7 +98,7 @@ def tune_trace(ctx, n: int, rays_ptr: int, hits_ptr: int, key: str, reps: int = 
     return best[0], table
 
 
-BATCH_CANDIDATES = tuple(dict(max_paths=1 << k, graph=g) for k in (20, 21, 22, 23) for g in (0, 1))
+BATCH_CANDIDATES = tuple(dict(max_paths=1 << k, graph=g) for k in (21, 22, 23, 24) for g in (0, 1))
 
 
 def tune_batch(ctx, jobs, rgba_ptr: int, key: str, sample: int = 48, reps: int = 2,

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AUR_ABI_VERSION 1u
+#define AUR_ABI_VERSION 2u
 
 typedef int32_t aur_status;
 enum {
@@ -198,13 +198,17 @@ aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
  * 65,536-ray tiles one sample at a time, src/render/monteCarloRaytracer.cpp:13).
  * max_paths: paths in flight per wavefront pass; a frame pass traces
  * clamp(max_paths / pixels, 1, remaining) samples of every pixel, a batch
- * pass max_paths paths.  0 selects the default (2^21).
+ * pass max_paths paths.  0 selects the default (2^24).
  * graph: 1 captures one pass (camera, bounce loop, accumulation) in a HIP
  * graph and replays it for every pass; 0 launches the kernels directly.
- * Images do not depend on either: samples are accumulated in sample order. */
+ * refill: the render's persistent traversal launches refill idle lanes once
+ * this many (1..64) are idle; 0 selects the default (16).
+ * Images do not depend on any of them: samples are accumulated in sample
+ * order. */
 typedef struct aur_render_params {
   uint32_t max_paths;
   uint32_t graph;
+  uint32_t refill;
 } aur_render_params;
 aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* params);
 aur_status aur_get_render_params(aur_context* ctx, aur_render_params* params);

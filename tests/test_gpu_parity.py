@@ -228,9 +228,10 @@ def test_render_progressive_equals_full(gpu_ctx):
     assert np.array_equal(full, split)
 
 
-@pytest.mark.parametrize("max_paths", [1, 3 * 64 * 64, 0])
-def test_render_pass_batching_invariant(gpu_ctx, max_paths):
-    """Samples per pass (paths in flight) must not change a single bit."""
+@pytest.mark.parametrize("max_paths,refill", [(1, 0), (3 * 64 * 64, 1), (0, 0), (0, 40)])
+def test_render_pass_batching_invariant(gpu_ctx, max_paths, refill):
+    """Samples per pass (paths in flight) and the traversal refill threshold
+    must not change a single bit."""
     s = scenes.cornell(width=64, height=64)
     _build(gpu_ctx, s)
     gpu_ctx.frame_create(64, 64, 8)
@@ -239,7 +240,7 @@ def test_render_pass_batching_invariant(gpu_ctx, max_paths):
         gpu_ctx.set_render_params(1)  # one sample per pass
         gpu_ctx.frame_render(s.camera, o)
         one = gpu_ctx.frame_read()
-        gpu_ctx.set_render_params(max_paths)
+        gpu_ctx.set_render_params(max_paths, 0, refill)
         gpu_ctx.frame_render(s.camera, o)
         many = gpu_ctx.frame_read()
     finally:
