@@ -347,7 +347,10 @@ struct PbrArgs {
   float4 *sh_o, *sh_d, *sh_c;
   int32_t* sh_occ;
   uint32_t* sh_queue;  // slot indices e = slot * cap + queue position with a shadow ray
-  uint32_t* counters;  // [0],[1] path queue lengths, [2] shadow queue length
+  // queue lengths as two 64-bit words, one per path queue q: counters[2q] =
+  // paths in queue q, counters[2q + 1] = shadow rays appended by the bounce
+  // that filled queue q (so shading appends to both with one atomic)
+  uint32_t* counters;
   unsigned long long* stats;
   // path-vertex records of an adjoint replay (null in a plain render),
   // indexed [bounce][q] and [bounce][slot][q] with stride cap: shader index of
@@ -399,6 +402,8 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   if (q == 0) {
     a.counters[0] = n;
     a.counters[1] = 0;
+    a.counters[2] = 0;
+    a.counters[3] = 0;
   }
   if (q >= n) return;
   const unsigned long long g = g0 + q;
@@ -447,13 +452,12 @@ template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
                                                        const uint32_t* __restrict__ queue) {
   extern __shared__ float2 stk_mem[];
-  const uint32_t n_in = a.counters[cin];
+  const uint32_t n_in = a.counters[2 * cin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the other queue's length and the shadow queue's were last read by the
-    // previous bounce's kernels, which have completed; the shade kernel that
-    // follows appends to both
-    a.counters[cin ^ 1] = 0;
-    a.counters[2] = 0;
+    // the other queue's counts were last read by the previous bounce's
+    // kernels, which have completed; the shade kernel that follows appends
+    a.counters[2 * (cin ^ 1)] = 0;
+    a.counters[2 * (cin ^ 1) + 1] = 0;
     atomicAdd(&a.stats[0], (unsigned long long)n_in);
   }
   persistent_trace<false>(
@@ -473,27 +477,18 @@ __global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
       [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
 }
 
-// Appends slot index e to the shadow queue: one atomic per wave for the lanes
-// appending together (a single-job pass visits lights in the same order in
-// every lane, so a wave's items stay grouped by slot).
-__device__ __forceinline__ void append_shadow(const PbrArgs& a, uint32_t e) {
-  const uint64_t m = __ballot(true);
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
-  const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t basepos = 0;
-  if (lane == leader) basepos = atomicAdd(&a.counters[2], (uint32_t)__popcll(m));
-  basepos = __shfl(basepos, leader);
-  a.sh_queue[basepos + rank] = e;
-}
+// What shading one path vertex leaves to the queues.
+struct ShadeOut {
+  bool survive;      // the path continues (next ray written)
+  uint64_t shadows;  // bit e: next-event slot e holds a shadow ray
+};
 
+// Shades queue entry j (path p): emission, next-event slots, Russian
+// roulette and the next direction.
 template <bool kMulti, bool kRecord>
-__global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t* __restrict__ qin,
-                            uint32_t* __restrict__ qout) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n_in = a.counters[cin];
-  if (j >= n_in) return;
-  const uint32_t p = qin[j];  // path index in the pass
+__device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uint32_t p,
+                                               uint32_t bounce) {
+  ShadeOut out{false, 0ull};
   const JobDev& J = job_of<kMulti>(a, p);
   const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
   const uint32_t pix = frame_pixel(J, r0 % J.npix), smp = r0 / J.npix;
@@ -503,7 +498,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   if (hit.id < 0) {
     for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s * ss].w = 0.0f;
     if (kRecord) a.rec_mat[bounce * np + p] = -1;
-    return;
+    return out;
   }
   const uint32_t sid = job_shader(J, (uint32_t)hit.id);
   if (kRecord) a.rec_mat[bounce * np + p] = (int32_t)sid;
@@ -547,7 +542,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
           a.sh_o[base + e * ss] = make_float4(so.x, so.y, so.z, dist + kEpsilon);
           a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, wgt * kInvPi);
           flag = 1.0f;
-          append_shadow(a, (uint32_t)(e * ss + base));
+          out.shadows |= 1ull << e;
         }
         a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       }
@@ -562,7 +557,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
         a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, dist);
         a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS / (dist * dist) * kInvPi);
         flag = 1.0f;
-        append_shadow(a, (uint32_t)(e * ss + base));
+        out.shadows |= 1ull << e;
       }
       a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
@@ -575,7 +570,7 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
         a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, kInfinity);
         a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, cosS * kInvPi);
         flag = 1.0f;
-        append_shadow(a, (uint32_t)(e * ss + base));
+        out.shadows |= 1ull << e;
       }
       a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
@@ -589,14 +584,14 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   if constexpr (kMulti) {
     for (; e < a.nslots; ++e) a.sh_d[base + e * ss].w = 0.0f;  // slots of jobs with more lights
   }
-  if (bounce + 1 >= J.max_depth) return;
+  if (bounce + 1 >= J.max_depth) return out;
   const U4 r = pbr_draw(J.seed, pix, smp, bounce, 1);
   if (kRecord) a.rec_rr[bounce * np + p] = 1.0f;
   if (bounce >= J.rr_start) {
     const float q = fminf(0.95f, fmaxf(beta.x, fmaxf(beta.y, beta.z)));
     if (u01(r.z) >= q) {
       if (kRecord) a.rec_mat[(bounce + 1) * np + p] = -1;
-      return;
+      return out;
     }
     beta = beta / q;
     if (kRecord) a.rec_rr[bounce * np + p] = q;
@@ -611,15 +606,45 @@ __global__ void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce, const uint32_t*
   a.ray_o[p] = make_float4(P.x, P.y, P.z, 0.0f);
   a.ray_d[p] = make_float4(wi.x, wi.y, wi.z, 0.0f);
   a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.0f);
-  // compaction: one atomic per wave, lanes ranked by ballot
-  const uint64_t m = __ballot(true);
+  out.survive = true;
+  return out;
+}
+
+// Shading of every queued path, then the two appends with one atomic per
+// wave each, made where every lane of the wave is active: the shadow queue
+// (slot-major within the wave, so neighbouring items trace to the same
+// light) and the next bounce's path queue.
+template <bool kMulti, bool kRecord>
+__global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce,
+                                                   const uint32_t* __restrict__ qin,
+                                                   uint32_t* __restrict__ qout) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_in = a.counters[2 * cin];
+  ShadeOut out{false, 0ull};
+  uint32_t p = 0;
+  if (j < n_in) {
+    p = qin[j];  // path index in the pass
+    out = shade_path<kMulti, kRecord>(a, j, p, bounce);
+  }
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
-  const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t basepos = 0;
-  if (lane == leader) basepos = atomicAdd(&a.counters[cin ^ 1], (uint32_t)__popcll(m));
-  basepos = __shfl(basepos, leader);
-  qout[basepos + rank] = p;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t nshadow = 0;
+  for (uint32_t sl = 0; sl < a.nslots; ++sl) nshadow += (uint32_t)__popcll(__ballot((out.shadows >> sl) & 1ull));
+  const uint64_t ms = __ballot(out.survive);
+  if (nshadow == 0 && ms == 0) return;
+  unsigned long long old = 0;
+  if (lane == 0)
+    old = atomicAdd(reinterpret_cast<unsigned long long*>(a.counters + 2 * (cin ^ 1)),
+                    (unsigned long long)__popcll(ms) | ((unsigned long long)nshadow << 32));
+  old = __shfl(old, 0);
+  if (out.survive) qout[(uint32_t)old + (uint32_t)__popcll(ms & below)] = p;
+  uint32_t qbase = (uint32_t)(old >> 32);
+  for (uint32_t sl = 0; sl < a.nslots; ++sl) {
+    const bool mine = (out.shadows >> sl) & 1ull;
+    const uint64_t m = __ballot(mine);
+    if (mine) a.sh_queue[qbase + (uint32_t)__popcll(m & below)] = sl * a.cap + j;
+    qbase += (uint32_t)__popcll(m);
+  }
 }
 
 // Any hit for every queued shadow ray (the slots shading filled).
@@ -627,7 +652,7 @@ template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
                                                        const uint32_t* __restrict__ qin) {
   extern __shared__ float2 stk_mem[];
-  const uint32_t total = a.counters[2];
+  const uint32_t total = a.counters[2 * (cin ^ 1) + 1];
   persistent_trace<true>(
       TreeRef{a.job.nodes, a.job.n_nodes}, total, a.refill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree) {
@@ -649,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
 template <bool kRecord>
 __global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_t* __restrict__ qin) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.counters[cin]) return;
+  if (j >= a.counters[2 * cin]) return;
   const uint32_t p = qin[j];
   if (a.hits[p].id < 0) return;
   const size_t np = a.cap;
@@ -961,6 +986,8 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   FrameState& fr = ctx->frame;
   hipStream_t s = ctx->stream;
   const uint32_t cap = run.cap;
+  if (run.nslots > 64)
+    return ctx->fail(AUR_ERR_UNSUPPORTED, "render: more than 64 next-event slots (light samples) per vertex");
   aur_status st = pbr_reserve(ctx, cap, run.nslots, run.max_depth, adj != nullptr);
   if (st != AUR_OK) return st;
   PbrArgs a{};
