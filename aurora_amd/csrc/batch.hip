@@ -138,8 +138,8 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
   std::vector<float> pass_ms;
   if (job_ms) run.pass_end_ms = &pass_ms;
   aur::FrameState& fr = ctx->frame;
-  AUR_HIP(ctx, fr.counters.reserve(8));
-  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
+  AUR_HIP(ctx, fr.counters.reserve(aur::kCounterWords));
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + aur::kStatsOffset);
   AUR_HIP(ctx, hipMemsetAsync(stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
   aur_status st = aur::pbr_run(ctx, run, stats);
   if (st != AUR_OK) return st;

@@ -53,6 +53,11 @@ struct DevBuf {
 // path state plus 48 B per next-event slot and path).
 constexpr uint32_t kDefaultMaxPaths = 1u << 24;
 constexpr uint32_t kDefaultRenderRefill = 16;
+// Frame counters (u32): three 64-bit queue-count words of the wavefront
+// passes, then two u64 ray counters (camera + extension, shadow) at
+// kStatsOffset.
+constexpr uint32_t kStatsOffset = 8;
+constexpr uint32_t kCounterWords = 12;
 
 // One render job of the PBR wavefront: a camera over one built scene.  A
 // frame render is a single job; a batch is many.  Paths are numbered
@@ -97,7 +102,7 @@ struct FrameState {
   DevBuf<float4> ray_o, ray_d, beta, radiance;
   DevBuf<aur_hit> hits;
   DevBuf<uint32_t> queue[2];
-  DevBuf<uint32_t> counters;  // [0..1] queue lengths, [2] shadow count, [3..] stats
+  DevBuf<uint32_t> counters;  // kCounterWords u32: queue counts, then ray counters
   DevBuf<float4> sh_o, sh_d, sh_c;  // shadow rays: origin+tmax, dir, contribution
   DevBuf<int32_t> sh_occ;
   DevBuf<uint32_t> sh_queue;  // slots holding a shadow ray, appended by shading

@@ -347,9 +347,11 @@ struct PbrArgs {
   float4 *sh_o, *sh_d, *sh_c;
   int32_t* sh_occ;
   uint32_t* sh_queue;  // slot indices e = slot * cap + queue position with a shadow ray
-  // queue lengths as two 64-bit words, one per path queue q: counters[2q] =
-  // paths in queue q, counters[2q + 1] = shadow rays appended by the bounce
-  // that filled queue q (so shading appends to both with one atomic)
+  // queue lengths as three 64-bit words used in turn, word k % 3 for bounce
+  // k: counters[2w] = paths entering bounce k, counters[2w + 1] = shadow rays
+  // shading appended at bounce k - 1 (shading appends to both with one
+  // atomic; three words keep bounce k - 1's counts for its resolve while
+  // bounce k + 1's are filled)
   uint32_t* counters;
   unsigned long long* stats;
   // path-vertex records of an adjoint replay (null in a plain render),
@@ -402,8 +404,6 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   if (q == 0) {
     a.counters[0] = n;
     a.counters[1] = 0;
-    a.counters[2] = 0;
-    a.counters[3] = 0;
   }
   if (q >= n) return;
   const unsigned long long g = g0 + q;
@@ -447,34 +447,64 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   queue[q] = q;
 }
 
-// Closest hit for every queued path: persistent waves over the queue.
+// One persistent launch for bounce k: closest hit for every path entering
+// bounce k (queue qext, items [0, n_ext)) and any hit for every shadow ray
+// shading appended at bounce k - 1 (the shadow queue, paths of queue qsh,
+// items [n_ext, n_ext + n_sh)); both counts in word w = k % 3.  parts bit 0:
+// extension rays, bit 1: shadow rays.  The two sets are independent, so one
+// launch overlaps their tails.
 template <bool kMulti>
-__global__ __launch_bounds__(kBlock) void k_pbr_extend(PbrArgs a, int cin,
-                                                       const uint32_t* __restrict__ queue) {
+__global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uint32_t parts,
+                                                      const uint32_t* __restrict__ qext,
+                                                      const uint32_t* __restrict__ qsh) {
   extern __shared__ float2 stk_mem[];
-  const uint32_t n_in = a.counters[2 * cin];
+  const uint32_t n_ext = (parts & 1u) ? a.counters[2 * w] : 0u;
+  const uint32_t n_sh = (parts & 2u) ? a.counters[2 * w + 1] : 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // the other queue's counts were last read by the previous bounce's
-    // kernels, which have completed; the shade kernel that follows appends
-    a.counters[2 * (cin ^ 1)] = 0;
-    a.counters[2 * (cin ^ 1) + 1] = 0;
-    atomicAdd(&a.stats[0], (unsigned long long)n_in);
+    // the next word was last read by the resolve of bounce k - 2, which has
+    // completed; the shade kernel that follows appends to it
+    const uint32_t wn = w == 2u ? 0u : w + 1u;
+    a.counters[2 * wn] = 0;
+    a.counters[2 * wn + 1] = 0;
+    atomicAdd(&a.stats[0], (unsigned long long)n_ext);
+    atomicAdd(&a.stats[1], (unsigned long long)n_sh);
   }
-  persistent_trace<false>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, n_in, a.refill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
-      [&](uint64_t j, V3& o, V3& d, float& tmax, TreeRef& tree) {
-        const uint32_t p = queue[j];
-        if constexpr (kMulti) {
-          const JobDev& J = a.jobs[a.path_job[p]];
-          tree = TreeRef{J.nodes, J.n_nodes};
+  persistent_trace(
+      TreeRef{a.job.nodes, a.job.n_nodes}, (uint64_t)n_ext + n_sh, a.refill,
+      lds_stack(stk_mem + threadIdx.x, blockDim.x),
+      [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree, bool& any) {
+        float4 o4, d4;
+        if (i < n_ext) {
+          const uint32_t p = qext[i];
+          if constexpr (kMulti) {
+            const JobDev& J = a.jobs[a.path_job[p]];
+            tree = TreeRef{J.nodes, J.n_nodes};
+          }
+          o4 = a.ray_o[p];
+          d4 = a.ray_d[p];
+          o4.w = kInfinity;
+          any = false;
+        } else {
+          const uint32_t e = a.sh_queue[i - n_ext];
+          if constexpr (kMulti) {
+            const JobDev& J = a.jobs[a.path_job[qsh[e % a.cap]]];
+            tree = TreeRef{J.nodes, J.n_nodes};
+          }
+          o4 = a.sh_o[e];
+          d4 = a.sh_d[e];
+          any = true;
         }
-        const float4 o4 = a.ray_o[p], d4 = a.ray_d[p];
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
-        tmax = kInfinity;
+        tmax = o4.w;
         return true;
       },
-      [&](uint64_t j, const Trav& s) { a.hits[queue[j]] = aur_hit{s.id, s.t, s.u, s.v}; });
+      [&](uint64_t i, const Trav& st) {
+        if (i < n_ext)
+          a.hits[qext[i]] = aur_hit{st.id, st.t, st.u, st.v};
+        else
+          a.sh_occ[a.sh_queue[i - n_ext]] = st.id >= 0 ? 1 : 0;
+      });
 }
 
 // What shading one path vertex leaves to the queues.
@@ -615,11 +645,11 @@ __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uin
 // (slot-major within the wave, so neighbouring items trace to the same
 // light) and the next bounce's path queue.
 template <bool kMulti, bool kRecord>
-__global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, int cin, uint32_t bounce,
+__global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32_t bounce,
                                                    const uint32_t* __restrict__ qin,
                                                    uint32_t* __restrict__ qout) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n_in = a.counters[2 * cin];
+  const uint32_t n_in = a.counters[2 * w];
   ShadeOut out{false, 0ull};
   uint32_t p = 0;
   if (j < n_in) {
@@ -634,7 +664,7 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, int cin, uint32_t 
   if (nshadow == 0 && ms == 0) return;
   unsigned long long old = 0;
   if (lane == 0)
-    old = atomicAdd(reinterpret_cast<unsigned long long*>(a.counters + 2 * (cin ^ 1)),
+    old = atomicAdd(reinterpret_cast<unsigned long long*>(a.counters + 2 * (w == 2u ? 0u : w + 1u)),
                     (unsigned long long)__popcll(ms) | ((unsigned long long)nshadow << 32));
   old = __shfl(old, 0);
   if (out.survive) qout[(uint32_t)old + (uint32_t)__popcll(ms & below)] = p;
@@ -647,36 +677,14 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, int cin, uint32_t 
   }
 }
 
-// Any hit for every queued shadow ray (the slots shading filled).
-template <bool kMulti>
-__global__ __launch_bounds__(kBlock) void k_pbr_shadow(PbrArgs a, int cin,
-                                                       const uint32_t* __restrict__ qin) {
-  extern __shared__ float2 stk_mem[];
-  const uint32_t total = a.counters[2 * (cin ^ 1) + 1];
-  persistent_trace<true>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, total, a.refill, lds_stack(stk_mem + threadIdx.x, blockDim.x),
-      [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree) {
-        const uint32_t e = a.sh_queue[i];
-        if constexpr (kMulti) {
-          const JobDev& J = a.jobs[a.path_job[qin[e % a.cap]]];
-          tree = TreeRef{J.nodes, J.n_nodes};
-        }
-        const float4 o4 = a.sh_o[e], d4 = a.sh_d[e];
-        o = mk(o4.x, o4.y, o4.z);
-        d = mk(d4.x, d4.y, d4.z);
-        tmax = o4.w;
-        return true;
-      },
-      [&](uint64_t i, const Trav& s) { a.sh_occ[a.sh_queue[i]] = s.id >= 0 ? 1 : 0; });
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.stats[1], (unsigned long long)total);
-}
-
+// Adds the unoccluded next-event contributions of bounce `bounce` (queue qin,
+// count in word w).  Paths without a hit at this bounce have every slot
+// flag 0 (hits[] may already hold the next bounce's result).
 template <bool kRecord>
-__global__ void k_pbr_resolve(PbrArgs a, int cin, uint32_t bounce, const uint32_t* __restrict__ qin) {
+__global__ void k_pbr_resolve(PbrArgs a, uint32_t w, uint32_t bounce, const uint32_t* __restrict__ qin) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.counters[2 * cin]) return;
+  if (j >= a.counters[2 * w]) return;
   const uint32_t p = qin[j];
-  if (a.hits[p].id < 0) return;
   const size_t np = a.cap;
   float4 L4 = a.radiance[p];
   V3 L = mk(L4.x, L4.y, L4.z);
@@ -918,7 +926,7 @@ aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t
   if ((uint64_t)cap * std::max<uint32_t>(nslots, 1u) > 0xFFFFFFFFull)
     return ctx->fail(AUR_ERR_INVALID_ARG, "render: too many paths per pass");
   const size_t nsh = std::max<size_t>(1, (size_t)cap * nslots);
-  AUR_HIP(ctx, fr.counters.reserve(8));
+  AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
   AUR_HIP(ctx, fr.pass_g0.reserve(1));
   AUR_HIP(ctx, fr.ray_o.reserve(cap));
   AUR_HIP(ctx, fr.ray_d.reserve(cap));
@@ -943,29 +951,32 @@ aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t
 // Kernels of one pass (camera, bounce loop, accumulation or adjoint sweep,
 // advance) on the context stream.
 template <bool kMulti>
-void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_extend, uint32_t grid_shadow,
+void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
               AdjointState* adj, uint32_t sweep_grid) {
   FrameState& fr = ctx->frame;
   hipStream_t s = ctx->stream;
   const uint32_t cap = a.cap;
   k_pbr_camera<kMulti><<<nblocks(cap, 256), 256, 0, s>>>(a, fr.queue[0].p);
-  int cin = 0;
+  // bounce b: shade, then one launch tracing bounce b + 1's extension rays
+  // with bounce b's shadow rays, then bounce b's next-event resolve
+  const uint32_t sh = a.nslots ? 2u : 0u;
+  k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, 0u, 1u, fr.queue[0].p, fr.queue[1].p);
   for (uint32_t b = 0; b < a.max_depth; ++b) {
-    uint32_t* qin = fr.queue[cin].p;
-    uint32_t* qout = fr.queue[cin ^ 1].p;
-    k_pbr_extend<kMulti><<<grid_extend, kBlock, lds, s>>>(a, cin, qin);
+    const uint32_t w = b % 3u, wn = (b + 1u) % 3u;
+    uint32_t* qin = fr.queue[b & 1u].p;
+    uint32_t* qout = fr.queue[(b + 1u) & 1u].p;
     if (adj)
-      k_pbr_shade<kMulti, true><<<nblocks(cap, 128), 128, 0, s>>>(a, cin, b, qin, qout);
+      k_pbr_shade<kMulti, true><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout);
     else
-      k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, cin, b, qin, qout);
+      k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout);
+    const uint32_t parts = (b + 1u < a.max_depth ? 1u : 0u) | sh;
+    if (parts) k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, wn, parts, qout, qin);
     if (a.nslots) {
-      k_pbr_shadow<kMulti><<<grid_shadow, kBlock, lds, s>>>(a, cin, qin);
       if (adj)
-        k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, cin, b, qin);
+        k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, w, b, qin);
       else
-        k_pbr_resolve<false><<<nblocks(cap, 256), 256, 0, s>>>(a, cin, b, qin);
+        k_pbr_resolve<false><<<nblocks(cap, 256), 256, 0, s>>>(a, w, b, qin);
     }
-    cin ^= 1;
   }
   if (!adj) {
     k_pbr_accumulate<kMulti><<<nblocks(cap, 256), 256, 0, s>>>(a);
@@ -1027,9 +1038,8 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     a.rec_w = fr.rec_w.p;
   }
   const size_t lds = (size_t)run.levels * kBlock * sizeof(float2);
-  const uint32_t grid_extend = resident_grid(ctx, (const void*)k_pbr_extend<kMulti>, lds, cap);
-  const uint32_t grid_shadow =
-      resident_grid(ctx, (const void*)k_pbr_shadow<kMulti>, lds, (uint64_t)cap * run.nslots);
+  const uint32_t grid_trace =
+      resident_grid(ctx, (const void*)k_pbr_trace<kMulti>, lds, (uint64_t)cap * (1u + run.nslots));
   uint32_t sweep_grid = 0;
   if (adj) {
     // a few resident blocks per CU; each keeps LDS accumulators for all parameters
@@ -1052,7 +1062,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      pbr_pass<kMulti>(ctx, a, lds, grid_extend, grid_shadow, adj, sweep_grid);
+      pbr_pass<kMulti>(ctx, a, lds, grid_trace, adj, sweep_grid);
       e = hipStreamEndCapture(s, &graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1064,7 +1074,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
       const hipError_t e = hipGraphLaunch(exec, s);
       if (e != hipSuccess) result = ctx->hip_fail(e, "hipGraphLaunch");
     } else {
-      pbr_pass<kMulti>(ctx, a, lds, grid_extend, grid_shadow, adj, sweep_grid);
+      pbr_pass<kMulti>(ctx, a, lds, grid_trace, adj, sweep_grid);
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) result = ctx->hip_fail(e, "pbr kernels");
     }
@@ -1186,8 +1196,8 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   const uint32_t w = fr.width, h = fr.height;
   const uint32_t rw = fr.right - fr.left + 1, rh = fr.top - fr.bottom + 1, npix = rw * rh;
   if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
-  AUR_HIP(ctx, fr.counters.reserve(8));
-  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
+  AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + kStatsOffset);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   if (o->mode == AUR_MODE_COMPAT) {
     SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
@@ -1244,8 +1254,8 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   const uint32_t npix = (fr.right - fr.left + 1) * (fr.top - fr.bottom + 1);  // region pixels
   const uint32_t nframe = fr.width * fr.height;
   if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: pixels * spp exceeds 2^32");
-  AUR_HIP(ctx, fr.counters.reserve(8));
-  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + 4);
+  AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + kStatsOffset);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   AdjointState adj;
   adj.dimg = reinterpret_cast<const float4*>(d_image);

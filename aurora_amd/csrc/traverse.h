@@ -131,6 +131,7 @@ struct Trav {
   int32_t id;
   uint32_t k, depth, trail;
   uint32_t axis;   // depth % 3
+  bool any;        // any-hit query (persistent_trace's mixed launches)
   float2 r;        // range of node k after its slab test
 };
 
@@ -217,6 +218,7 @@ __device__ __forceinline__ bool trav_init_root(Trav& s, float2 root, V3 o, V3 d,
   s.u = s.v = 0.0f;
   s.id = -1;
   s.k = s.depth = s.trail = s.axis = 0;
+  s.any = false;
   if (kCount) cnt->popped += 1;  // the root
   s.r = narrow(make_float2(kEpsilon, kInfinity), root.x, root.y, o.x, d.x, s.y.x, s.fast & 1u);
   return (s.r.x <= s.r.y) && (s.r.x <= s.t);
@@ -234,9 +236,11 @@ __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ n
 // the next node is chosen by selects: with several early exits the compiler's
 // structured control flow ran the node visit separately for lanes arriving
 // from different paths, doubling the instructions issued per visit.
-template <bool kAny, bool kCount, class Stk>
+// kMixed: the query kind is the lane's s.any instead of kAny.
+template <bool kAny, bool kCount, bool kMixed = false, class Stk>
 __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ nodes,
                                           uint32_t n_nodes, const Stk& stk, TraceCount* cnt) {
+  const bool any = kAny || (kMixed && s.any);
   const uint32_t k = s.k;
   if (kCount) {
     cnt->passed += 1;
@@ -268,14 +272,14 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     s.id = (int32_t)(2 * k);
   }
   const bool hitb = tri_finish(s.o, s.d, v1b, e1b, e2b, Pb, detb, ib, ub, vb, tb) &
-                    (tb > kEpsilon) & (tb < s.t) & !(kAny & hita);
+                    (tb > kEpsilon) & (tb < s.t) & !(any & hita);
   if (hitb) {
     s.t = tb;
     s.u = ub;
     s.v = vb;
     s.id = (int32_t)(2 * k + 1);
   }
-  const bool stop = kAny & (hita | hitb);  // any hit found
+  const bool stop = any & (hita | hitb);  // any hit found
   bool found = false;
   uint32_t nk = 0, nd = 0, na = 0;
   float2 nrg = s.r;
@@ -338,16 +342,18 @@ struct TreeRef {
   uint32_t n_nodes;
 };
 
-// Persistent-wave traversal of a work range [0, n).  Wave w owns
-// [w*chunk, (w+1)*chunk); between node visits, once at least `refill` lanes
-// are idle (or all are), idle lanes take the next items of the range, ranked
-// by ballot (no atomics).  fetch(i, o, d, tmax, tree) loads item i, may
-// replace the lane's tree (initially tree0; batches of scenes), and returns
-// false if the item needs no traversal; store(i, s) receives the finished
-// state.  A fetch that never writes `tree` keeps the tree wave-uniform.
-// (Splitting a refill over two iterations, so that the item loads overlap
-// the active lanes' node step, measured slower: the new lanes lose a step.)
-template <bool kAny, class Stk, class Fetch, class Store>
+// Persistent-wave traversal of a work range [0, n) of closest-hit and
+// any-hit queries.  Wave w owns [w*chunk, (w+1)*chunk); between node visits,
+// once at least `refill` lanes are idle (or all are), idle lanes take the
+// next items of the range, ranked by ballot (no atomics).
+// fetch(i, o, d, tmax, tree, any) loads item i, may replace the lane's tree
+// (initially tree0; batches of scenes), sets any for an any-hit query and
+// returns false if the item needs no traversal; store(i, s) receives the
+// finished state.  A fetch that never writes `tree` keeps the tree
+// wave-uniform.  (Splitting a refill over two iterations, so that the item
+// loads overlap the active lanes' node step, measured slower: the new lanes
+// lose a step.)
+template <class Stk, class Fetch, class Store>
 __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint32_t refill,
                                                  const Stk& stk, Fetch fetch, Store store) {
   TreeRef tree = tree0;
@@ -371,9 +377,11 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint
           if (i < end) {
             V3 o, d;
             float tmax;
-            if (fetch(i, o, d, tmax, tree)) {
+            bool any = false;
+            if (fetch(i, o, d, tmax, tree, any)) {
               item = i;
               active = trav_init<false>(s, tree.nodes, tree.n_nodes, o, d, tmax, nullptr);
+              s.any = any;
               if (!active) store(i, s);
             }
           }
@@ -383,7 +391,7 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint
         break;
       }
     }
-    if (active && trav_step<kAny, false>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
+    if (active && trav_step<false, false, true>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
       store(item, s);
       active = false;
     }
