@@ -579,7 +579,10 @@ def workload_batch_c4(a) -> dict:
         o0 = int(jobs[k]["out_offset"])
         n = sc[k].width * sc[k].height
         same += int(np.array_equal(img[o0:o0 + n], images[k]))
-    log(f"identical images: {same}/{len(check)}")
+    if a.batch_only:
+        log("identical images: not checked (--batch-only renders no single frames)")
+    else:
+        log(f"identical images: {same}/{len(check)}")
     ctx.close()
     lat_batch = np.asarray(job_ms, dtype=np.float64) / 1e3
     speedup = T_single / T_batch
