@@ -342,7 +342,10 @@ struct PbrArgs {
   unsigned long long end;
   uint32_t cap, nslots, max_depth;
   uint32_t refill;  // persistent traversal: refill once this many lanes are idle
-  float4 *ray_o, *ray_d, *beta, *radiance;
+  // rays of bounce k in queue order at [(k & 1) * cap + queue position];
+  // hits of the current bounce in queue order
+  float4 *ray_o, *ray_d;
+  float4 *beta, *radiance;
   aur_hit* hits;
   float4 *sh_o, *sh_d, *sh_c;
   int32_t* sh_occ;
@@ -448,13 +451,15 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
 }
 
 // One persistent launch for bounce k: closest hit for every path entering
-// bounce k (queue qext, items [0, n_ext)) and any hit for every shadow ray
+// bounce k (rays in queue order from ray_o/ray_d + roff, paths in queue qext,
+// items [0, n_ext); hits in queue order) and any hit for every shadow ray
 // shading appended at bounce k - 1 (the shadow queue, paths of queue qsh,
 // items [n_ext, n_ext + n_sh)); both counts in word w = k % 3.  parts bit 0:
 // extension rays, bit 1: shadow rays.  The two sets are independent, so one
 // launch overlaps their tails.
 template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uint32_t parts,
+                                                      uint32_t roff,
                                                       const uint32_t* __restrict__ qext,
                                                       const uint32_t* __restrict__ qsh) {
   extern __shared__ float2 stk_mem[];
@@ -469,19 +474,18 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
     atomicAdd(&a.stats[0], (unsigned long long)n_ext);
     atomicAdd(&a.stats[1], (unsigned long long)n_sh);
   }
-  persistent_trace(
+  persistent_trace<!kMulti>(
       TreeRef{a.job.nodes, a.job.n_nodes}, (uint64_t)n_ext + n_sh, a.refill,
       lds_stack(stk_mem + threadIdx.x, blockDim.x),
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree, bool& any) {
         float4 o4, d4;
         if (i < n_ext) {
-          const uint32_t p = qext[i];
           if constexpr (kMulti) {
-            const JobDev& J = a.jobs[a.path_job[p]];
+            const JobDev& J = a.jobs[a.path_job[qext[i]]];
             tree = TreeRef{J.nodes, J.n_nodes};
           }
-          o4 = a.ray_o[p];
-          d4 = a.ray_d[p];
+          o4 = a.ray_o[roff + i];
+          d4 = a.ray_d[roff + i];
           o4.w = kInfinity;
           any = false;
         } else {
@@ -501,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
       },
       [&](uint64_t i, const Trav& st) {
         if (i < n_ext)
-          a.hits[qext[i]] = aur_hit{st.id, st.t, st.u, st.v};
+          a.hits[i] = aur_hit{st.id, st.t, st.u, st.v};
         else
           a.sh_occ[a.sh_queue[i - n_ext]] = st.id >= 0 ? 1 : 0;
       });
@@ -509,22 +513,23 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
 
 // What shading one path vertex leaves to the queues.
 struct ShadeOut {
-  bool survive;      // the path continues (next ray written)
+  bool survive;      // the path continues with ray (o, d)
   uint64_t shadows;  // bit e: next-event slot e holds a shadow ray
+  float4 o, d;
 };
 
 // Shades queue entry j (path p): emission, next-event slots, Russian
 // roulette and the next direction.
 template <bool kMulti, bool kRecord>
 __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uint32_t p,
-                                               uint32_t bounce) {
-  ShadeOut out{false, 0ull};
+                                               uint32_t bounce, uint32_t roff) {
+  ShadeOut out{false, 0ull, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
   const JobDev& J = job_of<kMulti>(a, p);
   const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
   const uint32_t pix = frame_pixel(J, r0 % J.npix), smp = r0 / J.npix;
   const size_t base = j, ss = a.cap;  // slot e of queue entry j at base + e * ss
   const size_t np = a.cap;
-  const aur_hit hit = a.hits[p];
+  const aur_hit hit = a.hits[j];
   if (hit.id < 0) {
     for (uint32_t s = 0; s < a.nslots; ++s) a.sh_d[base + s * ss].w = 0.0f;
     if (kRecord) a.rec_mat[bounce * np + p] = -1;
@@ -532,7 +537,7 @@ __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uin
   }
   const uint32_t sid = job_shader(J, (uint32_t)hit.id);
   if (kRecord) a.rec_mat[bounce * np + p] = (int32_t)sid;
-  const float4 o4 = a.ray_o[p], d4 = a.ray_d[p], b4 = a.beta[p];
+  const float4 o4 = a.ray_o[roff + j], d4 = a.ray_d[roff + j], b4 = a.beta[p];
   float4 L4 = a.radiance[p];
   const V3 O = mk(o4.x, o4.y, o4.z), D = mk(d4.x, d4.y, d4.z);
   V3 beta = mk(b4.x, b4.y, b4.z);
@@ -633,28 +638,29 @@ __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uin
   onb(N, t1, t2);
   const V3 wi = t1 * dx + t2 * dy + N * dz;
   beta = beta * albedo;
-  a.ray_o[p] = make_float4(P.x, P.y, P.z, 0.0f);
-  a.ray_d[p] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+  out.o = make_float4(P.x, P.y, P.z, 0.0f);
+  out.d = make_float4(wi.x, wi.y, wi.z, 0.0f);
   a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.0f);
   out.survive = true;
   return out;
 }
 
-// Shading of every queued path, then the two appends with one atomic per
-// wave each, made where every lane of the wave is active: the shadow queue
-// (slot-major within the wave, so neighbouring items trace to the same
-// light) and the next bounce's path queue.
+// Shading of every queued path, then the appends with one atomic per wave,
+// made where every lane of the wave is active: the next bounce's path queue
+// (with its rays, in queue order) and the shadow queue (slot-major within
+// the wave, so neighbouring items trace to the same light).
 template <bool kMulti, bool kRecord>
 __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32_t bounce,
                                                    const uint32_t* __restrict__ qin,
                                                    uint32_t* __restrict__ qout) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_in = a.counters[2 * w];
-  ShadeOut out{false, 0ull};
+  const uint32_t roff = (bounce & 1u) * a.cap, roff_next = a.cap - roff;
+  ShadeOut out{false, 0ull, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
   uint32_t p = 0;
   if (j < n_in) {
     p = qin[j];  // path index in the pass
-    out = shade_path<kMulti, kRecord>(a, j, p, bounce);
+    out = shade_path<kMulti, kRecord>(a, j, p, bounce, roff);
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -667,7 +673,12 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32
     old = atomicAdd(reinterpret_cast<unsigned long long*>(a.counters + 2 * (w == 2u ? 0u : w + 1u)),
                     (unsigned long long)__popcll(ms) | ((unsigned long long)nshadow << 32));
   old = __shfl(old, 0);
-  if (out.survive) qout[(uint32_t)old + (uint32_t)__popcll(ms & below)] = p;
+  if (out.survive) {
+    const uint32_t pos = (uint32_t)old + (uint32_t)__popcll(ms & below);
+    qout[pos] = p;
+    a.ray_o[roff_next + pos] = out.o;
+    a.ray_d[roff_next + pos] = out.d;
+  }
   uint32_t qbase = (uint32_t)(old >> 32);
   for (uint32_t sl = 0; sl < a.nslots; ++sl) {
     const bool mine = (out.shadows >> sl) & 1ull;
@@ -928,8 +939,8 @@ aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t
   const size_t nsh = std::max<size_t>(1, (size_t)cap * nslots);
   AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
   AUR_HIP(ctx, fr.pass_g0.reserve(1));
-  AUR_HIP(ctx, fr.ray_o.reserve(cap));
-  AUR_HIP(ctx, fr.ray_d.reserve(cap));
+  AUR_HIP(ctx, fr.ray_o.reserve((size_t)cap * 2));
+  AUR_HIP(ctx, fr.ray_d.reserve((size_t)cap * 2));
   AUR_HIP(ctx, fr.beta.reserve(cap));
   AUR_HIP(ctx, fr.radiance.reserve(cap));
   AUR_HIP(ctx, fr.hits.reserve(cap));
@@ -960,7 +971,7 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
   // bounce b: shade, then one launch tracing bounce b + 1's extension rays
   // with bounce b's shadow rays, then bounce b's next-event resolve
   const uint32_t sh = a.nslots ? 2u : 0u;
-  k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, 0u, 1u, fr.queue[0].p, fr.queue[1].p);
+  k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, 0u, 1u, 0u, fr.queue[0].p, fr.queue[1].p);
   for (uint32_t b = 0; b < a.max_depth; ++b) {
     const uint32_t w = b % 3u, wn = (b + 1u) % 3u;
     uint32_t* qin = fr.queue[b & 1u].p;
@@ -970,7 +981,8 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
     else
       k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout);
     const uint32_t parts = (b + 1u < a.max_depth ? 1u : 0u) | sh;
-    if (parts) k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, wn, parts, qout, qin);
+    if (parts)
+      k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, wn, parts, ((b + 1u) & 1u) * cap, qout, qin);
     if (a.nslots) {
       if (adj)
         k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, w, b, qin);
