@@ -109,6 +109,12 @@ aur_status aur_synchronize(aur_context* ctx) {
   return AUR_OK;
 }
 
+aur_status aur_cancel(aur_context* ctx) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  ctx->cancel.store(1u, std::memory_order_release);
+  return AUR_OK;
+}
+
 aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* vertices,
                                const float* normals, const float* texcoords,
                                const uint16_t* shader_ids) {

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdio>
 #include <memory>
 #include <string>
@@ -133,6 +134,7 @@ struct aur_context {
   aur_trace_params trace_params{1, 256, 24, 0, 0};
   aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
   aur_render_params render_params{aur::kDefaultMaxPaths, 0, aur::kDefaultRenderRefill};
+  std::atomic<uint32_t> cancel{0};  // set by aur_cancel, consumed by the render it stops
 
   // uploaded geometry (input order)
   uint32_t count = 0;
@@ -170,6 +172,8 @@ struct aur_context {
     last_error = msg;
     return code;
   }
+  // true once per aur_cancel request
+  bool take_cancel() { return cancel.exchange(0u, std::memory_order_acq_rel) != 0u; }
   aur_status hip_fail(hipError_t e, const char* what) {
     last_error = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? AUR_ERR_OOM : AUR_ERR_DEVICE;
@@ -181,6 +185,24 @@ struct aur_context {
     hipError_t _e = (expr);                                  \
     if (_e != hipSuccess) return (ctx)->hip_fail(_e, #expr); \
   } while (0)
+
+// Keeps at most two passes (or samples) queued ahead of the GPU, so that a
+// cancel request is seen within about two passes: wait(i) before queuing
+// pass i, mark(i) after it.
+struct PassThrottle {
+  hipStream_t stream;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  explicit PassThrottle(hipStream_t s) : stream(s) {
+    for (auto& e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+  }
+  ~PassThrottle() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t wait(uint64_t i) { return (i >= 2 && ev[i & 1]) ? hipEventSynchronize(ev[i & 1]) : hipSuccess; }
+  hipError_t mark(uint64_t i) { return ev[i & 1] ? hipEventRecord(ev[i & 1], stream) : hipSuccess; }
+};
 
 #define AUR_LAUNCH_CHECK(ctx, what)                          \
   do {                                                       \

@@ -1081,7 +1081,18 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) result = ctx->hip_fail(e, "pbr graph capture");
   }
+  PassThrottle throttle(s);
   for (uint64_t i = 0; result == AUR_OK && i < npass; ++i) {
+    hipError_t te = throttle.wait(i);
+    if (te != hipSuccess) {
+      result = ctx->hip_fail(te, "pass throttle");
+      break;
+    }
+    if (ctx->take_cancel()) {
+      (void)hipStreamSynchronize(s);
+      result = ctx->fail(AUR_ERR_CANCELLED, "render cancelled");
+      break;
+    }
     if (exec) {
       const hipError_t e = hipGraphLaunch(exec, s);
       if (e != hipSuccess) result = ctx->hip_fail(e, "hipGraphLaunch");
@@ -1094,6 +1105,8 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
       const hipError_t e = hipEventRecord(ev[i + 1], s);
       if (e != hipSuccess) result = ctx->hip_fail(e, "hipEventRecord");
     }
+    te = throttle.mark(i);
+    if (result == AUR_OK && te != hipSuccess) result = ctx->hip_fail(te, "pass throttle");
   }
   if (exec) {
     (void)hipStreamSynchronize(s);
@@ -1101,9 +1114,11 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   }
   if (run.pass_end_ms) {
     (void)hipStreamSynchronize(s);
-    run.pass_end_ms->assign(npass, 0.0f);
-    for (uint64_t i = 0; result == AUR_OK && i < npass; ++i)
-      (void)hipEventElapsedTime(&(*run.pass_end_ms)[i], ev[0], ev[i + 1]);
+    if (result == AUR_OK) {
+      run.pass_end_ms->assign(npass, 0.0f);
+      for (uint64_t i = 0; i < npass; ++i)
+        (void)hipEventElapsedTime(&(*run.pass_end_ms)[i], ev[0], ev[i + 1]);
+    }
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
   return result;
@@ -1217,11 +1232,18 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
                 ctx->texels.p, ctx->num_textures};
     const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
     const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
+    PassThrottle throttle(s);
     for (uint32_t smp = s0; smp < s1; ++smp) {
+      AUR_HIP(ctx, throttle.wait(smp - s0));
+      if (ctx->take_cancel()) {
+        AUR_HIP(ctx, hipStreamSynchronize(s));
+        return ctx->fail(AUR_ERR_CANCELLED, "render cancelled");
+      }
       k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, region, o->spp, smp, o->seed,
                                                           fr.pixels.p, stats);
+      AUR_LAUNCH_CHECK(ctx, "k_compat");
+      AUR_HIP(ctx, throttle.mark(smp - s0));
     }
-    AUR_LAUNCH_CHECK(ctx, "k_compat");
   } else if (s1 > s0) {
     const JobDev J = frame_job(ctx, cam, o);
     PbrRun run;

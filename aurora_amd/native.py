@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG, "libaurora_amd.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "aurora_amd.h")
 
 AUR_OK = 0
+AUR_ERR_CANCELLED = 4
 STATUS_NAMES = {0: "OK", 1: "OOM", 2: "INVALID_ARG", 3: "DEVICE", 4: "CANCELLED", 5: "UNSUPPORTED",
                 6: "STATE"}
 MODE_COMPAT, MODE_PBR = 0, 1
@@ -97,6 +98,7 @@ def load():
         "aur_abi_version": ([], u32),
         "aur_stream": ([P], P),
         "aur_synchronize": ([P], i32),
+        "aur_cancel": ([P], i32),
         "aur_geometry_upload": ([P, u32, P, P, P, P], i32),
         "aur_shaders_upload": ([P, u32, P], i32),
         "aur_lights_upload": ([P, u32, P], i32),
@@ -196,6 +198,12 @@ class Context:
 
     def synchronize(self) -> None:
         self._check(self._lib.aur_synchronize(self._h), "synchronize")
+
+    def cancel(self) -> None:
+        """Stops the render running on this context (another thread) at its
+        next pass boundary, or the next render at its first; that render
+        raises AuroraError with status CANCELLED."""
+        self._check(self._lib.aur_cancel(self._h), "cancel")
 
     # -- scene --
     def upload_geometry(self, vertices, normals=None, texcoords=None, shader_ids=None) -> None:

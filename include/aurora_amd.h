@@ -110,6 +110,17 @@ const char* aur_last_error(const aur_context* ctx);
 uint32_t aur_abi_version(void);
 void* aur_stream(aur_context* ctx); /* hipStream_t of the context */
 aur_status aur_synchronize(aur_context* ctx);
+/* Stops the render running on ctx at its next pass boundary (or, with none
+ * running, the next render at its first): that call returns
+ * AUR_ERR_CANCELLED, its frame holding the passes that completed, and the
+ * request is consumed.  The one call that may run concurrently with another
+ * call on the same context (from another thread).  Covers aur_frame_render,
+ * aur_frame_render_samples, aur_frame_adjoint and aur_batch_render; passes
+ * are queued at most two ahead of the GPU so that a cancel takes effect
+ * within about two passes (one sample per compat pass).  Replaces the
+ * reference's stop-on-interrupt of the IPR loop (src/core/engine.cpp:258-317,
+ * m_pause / MComputation::isInterruptRequested). */
+aur_status aur_cancel(aur_context* ctx);
 
 /* ---- scene upload (replaces Geometry::resize/copyToDevice and
  * Scene::updateShaders/updateLights uploads: src/data/geometry.cpp:88-151,
