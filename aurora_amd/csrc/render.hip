@@ -518,13 +518,12 @@ struct ShadeOut {
   float4 o, d;
 };
 
-// Shades queue entry j (path p): emission, next-event slots, Russian
-// roulette and the next direction.
+// Shades queue entry j (path p of job J): emission, next-event slots,
+// Russian roulette and the next direction.
 template <bool kMulti, bool kRecord>
-__device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uint32_t p,
-                                               uint32_t bounce, uint32_t roff) {
+__device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, const JobDev& J, uint32_t j,
+                                               uint32_t p, uint32_t bounce, uint32_t roff) {
   ShadeOut out{false, 0ull, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
-  const JobDev& J = job_of<kMulti>(a, p);
   const uint32_t r0 = (uint32_t)(*a.g0p + p - J.path_start);
   const uint32_t pix = frame_pixel(J, r0 % J.npix), smp = r0 / J.npix;
   const size_t base = j, ss = a.cap;  // slot e of queue entry j at base + e * ss
@@ -652,15 +651,38 @@ __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, uint32_t j, uin
 template <bool kMulti, bool kRecord>
 __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32_t bounce,
                                                    const uint32_t* __restrict__ qin,
-                                                   uint32_t* __restrict__ qout) {
+                                                   uint32_t* __restrict__ qout,
+                                                   const JobDev* __restrict__ jobs) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_in = a.counters[2 * w];
   const uint32_t roff = (bounce & 1u) * a.cap, roff_next = a.cap - roff;
   ShadeOut out{false, 0ull, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
   uint32_t p = 0;
-  if (j < n_in) {
-    p = qin[j];  // path index in the pass
-    out = shade_path<kMulti, kRecord>(a, j, p, bounce, roff);
+  if constexpr (kMulti) {
+    // one round per distinct job among the wave's paths (almost always one:
+    // paths are numbered job by job), each reading its job record at a
+    // wave-uniform address
+    const bool act = j < n_in;
+    uint32_t jb = 0;
+    if (act) {
+      p = qin[j];
+      jb = a.path_job[p];
+    }
+    uint64_t todo = __ballot(act);
+    while (todo) {
+      const uint32_t jbu = (uint32_t)__builtin_amdgcn_readlane((int)jb, (int)(__ffsll((unsigned long long)todo) - 1));
+      const bool mine = act && jb == jbu;
+      // read where the whole wave is active, through the read-only kernel
+      // argument: a wave-uniform record comes through scalar loads
+      const JobDev J = jobs[jbu];
+      if (mine) out = shade_path<kMulti, kRecord>(a, J, j, p, bounce, roff);
+      todo &= ~__ballot(mine);
+    }
+  } else {
+    if (j < n_in) {
+      p = qin[j];  // path index in the pass
+      out = shade_path<kMulti, kRecord>(a, a.job, j, p, bounce, roff);
+    }
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -977,9 +999,9 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
     uint32_t* qin = fr.queue[b & 1u].p;
     uint32_t* qout = fr.queue[(b + 1u) & 1u].p;
     if (adj)
-      k_pbr_shade<kMulti, true><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout);
+      k_pbr_shade<kMulti, true><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout, a.jobs);
     else
-      k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout);
+      k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout, a.jobs);
     const uint32_t parts = (b + 1u < a.max_depth ? 1u : 0u) | sh;
     if (parts)
       k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, wn, parts, ((b + 1u) & 1u) * cap, qout, qin);
