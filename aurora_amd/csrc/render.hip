@@ -1183,7 +1183,7 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
 // 3x3 tent filter of the region (cudaFilterPixelsKernel,
 // src/kernels/utility.cu:231-258): weights 1/16, 1/8, 1/4, taps clamped to
 // the region, sums in the reference's order.  The reference clamps y + 1 to
-// region.bottom (F9), which this build fixes to region.top.
+// region.bottom, a bug; here it is clamped to region.top.
 __global__ void k_filter_pixels(uint32_t w, int left, int right, int bottom, int top,
                                 const float4* __restrict__ in, float4* __restrict__ out) {
   const int x = left + (int)(blockIdx.x * blockDim.x + threadIdx.x);

@@ -409,7 +409,8 @@ def canonical(a: np.ndarray) -> np.ndarray:
 def cornell(seed: int = 1, n_spheres: int = 2, subdivisions: int = 3, width: int = 256,
             height: int = 256, spp: int = 16, max_depth: int = 4, n_lights: int = 1,
             split_materials: bool = False) -> Scene:
-    """Config c1 (and the c4/c5 family): Cornell-style [0,5]^3 box + icospheres."""
+    """Cornell-style [0,5]^3 box + icospheres (2 at subdivision 3 by default;
+    the batch and adjoint scenes use 2-40 at subdivision 4)."""
     rng = np.random.default_rng(seed)
     parts, ids, shaders = [], [], []
     for q in _box_quads(0.0, 5.0):
@@ -463,7 +464,7 @@ def cornell(seed: int = 1, n_spheres: int = 2, subdivisions: int = 3, width: int
 
 def spheres(seed: int = 2, n_spheres: int = 200, subdivisions: int = 4, box: float = 100.0,
             width: int = 1920, height: int = 1080, spp: int = 64, max_depth: int = 8) -> Scene:
-    """Config c2: icospheres in a box, ground plane, 4 quad lights, 32 materials."""
+    """200 icospheres in a 100 m box, ground plane, 4 quad lights, 32 materials."""
     rng = np.random.default_rng(seed)
     shaders = []
     for m in range(32):
@@ -500,7 +501,7 @@ def spheres(seed: int = 2, n_spheres: int = 200, subdivisions: int = 4, box: flo
 
 
 def soup(seed: int = 3, n: int = 10_000_000, box: float = 1000.0, chunk: int = 1 << 20) -> Scene:
-    """Config c3: random equilateral-triangle soup (centroids uniform in the box,
+    """Random equilateral-triangle soup (centroids uniform in the box,
     edge ~ LogNormal(0, 1) m, uniformly random orientation)."""
     rng = np.random.default_rng(seed)
     verts = np.empty((n, 9), dtype=np.float32)

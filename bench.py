@@ -296,11 +296,11 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
             "error_estimate": "two-seed: ||I7 - I8|| / (sqrt2 ||(I7 + I8)/2||)"}
 
 
-def workload_render_c1(a) -> dict:
+def workload_render_cornell(a) -> dict:
     from aurora_amd import scenes
 
     scene = scenes.cornell()  # 256^2, depth 4, RR from bounce 3
-    res = render_convergence(a, scene, "c1", 1 << 16)
+    res = render_convergence(a, scene, "cornell", 1 << 16)
     cpu = None
     if not a.no_cpu and res["to_1pct"]:
         from tests import oracle_ffi
@@ -315,23 +315,23 @@ def workload_render_c1(a) -> dict:
         cpu = {"value": round(dt, 2), "unit": "s", "cores": th, "kind": "port",
                "sample": f"oracle PBR render at {spp} spp, scaled to {res['to_1pct']['spp']} spp"}
     hit = res["to_1pct"]
-    return {"metric": "s to 1% rel-RMSE (c1 Cornell, PBR)", "value": hit["s"] if hit else None,
+    return {"metric": "s to 1% rel-RMSE (Cornell box + 2 spheres, 256x256, PBR)", "value": hit["s"] if hit else None,
             "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
             "data": "synthetic (seeded Cornell box + 2 icospheres)",
-            "config": {"workload": "c1: 256x256, depth 4, RR from bounce 3", **res},
+            "config": {"workload": "Cornell box + 2 subdivision-3 spheres (2,584 triangles), 256x256, depth 4, RR from bounce 3", **res},
             "cpu_baseline": cpu}
 
 
-def workload_render_c2(a) -> dict:
+def workload_render_spheres(a) -> dict:
     from aurora_amd import scenes
 
     scene = scenes.spheres()  # 1920x1080, depth 8
-    res = render_convergence(a, scene, "c2", 4096)
+    res = render_convergence(a, scene, "spheres", 4096)
     hit = res["to_1pct"]
-    return {"metric": "s to 1% rel-RMSE (c2 spheres 1080p, PBR)", "value": hit["s"] if hit else None,
+    return {"metric": "s to 1% rel-RMSE (200 spheres, 1M triangles, 1920x1080, PBR)", "value": hit["s"] if hit else None,
             "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
             "data": "synthetic (seeded 200 icospheres + ground, 4 quad lights)",
-            "config": {"workload": "c2: 1920x1080, depth 8, RR from bounce 3", **res}}
+            "config": {"workload": "200 subdivision-4 spheres + ground (1,024,002 triangles), 4 quad lights, 1920x1080, depth 8, RR from bounce 3", **res}}
 
 
 def c5_scene():
@@ -343,7 +343,7 @@ def c5_scene():
                           n_lights=4, split_materials=True)
 
 
-def workload_adjoint_c5(a) -> dict:
+def workload_adjoint_cornell(a) -> dict:
     """Adjoint (gradient of an L2 image loss w.r.t. 64 RGB albedos and 4 RGB
     light radiances) vs forward render at equal spp; central-difference
     check (h = 1e-3, common random numbers) of the largest gradients; 100
@@ -354,7 +354,7 @@ def workload_adjoint_c5(a) -> dict:
 
     scene = c5_scene()
     spp, depth = a.spp or 64, scene.max_depth
-    log(f"c5: {scene.count} triangles, {scene.shaders.shape[0]} shaders, {scene.lights.shape[0]} lights")
+    log(f"adjoint scene: {scene.count} triangles, {scene.shaders.shape[0]} shaders, {scene.lights.shape[0]} lights")
     r = diff.DiffRenderer(scene, spp=spp, max_depth=depth, seed=7)
     if a.max_paths:
         r.ctx.set_render_params(a.max_paths)
@@ -460,11 +460,11 @@ def workload_adjoint_c5(a) -> dict:
     r.close()
     ratio = t_adj / t_fwd
     return {
-        "metric": "adjoint/fwd time (c5)", "value": round(ratio, 3), "unit": "x",
+        "metric": "adjoint/fwd time (Cornell box + 20 spheres, 512x512)", "value": round(ratio, 3), "unit": "x",
         "n_gpus": 1, "steps": len(fw), "higher_is_better": False, "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded Cornell box + 20 icospheres, 64 shaders, 4 lights)",
         "config": {
-            "workload": "c5: 512x512, depth 6 (no Russian roulette), L2 loss vs 1024-spp target",
+            "workload": "Cornell box + 20 subdivision-4 spheres (102,424 triangles), 64 albedos + 4 emitters, 512x512, depth 6 (no Russian roulette), L2 loss vs 1024-spp target",
             "triangles": scene.count, "spp": spp, "forward_ms": round(t_fwd * 1e3, 2),
             "adjoint_ms": round(t_adj * 1e3, 2), "forward_mrays": round(rays_fwd / t_fwd / 1e6, 1),
             "fd_h": h, "fd_max_rel_err": max_fd_err, "fd": fd_rows,
@@ -498,7 +498,7 @@ def c4_jobs(n_jobs: int = 256, seed: int = 44):
     return out
 
 
-def workload_batch_c4(a) -> dict:
+def workload_batch256(a) -> dict:
     """256 jobs in shared wavefront passes (aur_batch_render) vs the same
     jobs rendered one by one with aur_frame_render (scenes built beforehand
     in both cases).  Images are compared bit for bit, so both sides reach the
@@ -511,7 +511,7 @@ def workload_batch_c4(a) -> dict:
     sc = c4_jobs()
     tri = np.array([s.count for s in sc])
     paths = np.array([s.width * s.height * s.spp for s in sc], dtype=np.int64)
-    log(f"c4: {len(sc)} jobs, {tri.min()}..{tri.max()} triangles, {paths.sum() / 1e6:.0f}M paths "
+    log(f"batch: {len(sc)} jobs, {tri.min()}..{tri.max()} triangles, {paths.sum() / 1e6:.0f}M paths "
         f"({time.perf_counter() - t0:.1f}s)")
     order = np.argsort(paths, kind="stable")  # both sides run shortest jobs first
 
@@ -561,7 +561,7 @@ def workload_batch_c4(a) -> dict:
     total_px = int(sum(s.width * s.height for s in sc))
     out = torch.zeros((total_px, 4), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
-    params, table = tune.tune_batch(ctx, jobs, out.data_ptr(), key="c4", force=a.tune)
+    params, table = tune.tune_batch(ctx, jobs, out.data_ptr(), key="batch256", force=a.tune)
     log(f"batch params {params}" + (f" (tuned: {[(p['max_paths'], p['graph'], round(ms, 1)) for p, ms in table]})"
                                     if table else " (cached)"))
     ctx.batch_render_ptr(jobs, out.data_ptr())  # warm-up
@@ -587,11 +587,11 @@ def workload_batch_c4(a) -> dict:
     lat_batch = np.asarray(job_ms, dtype=np.float64) / 1e3
     speedup = T_single / T_batch
     return {
-        "metric": "batched jobs/s (c4, 256 jobs)", "value": round(len(sc) / T_batch, 2), "unit": "jobs/s",
+        "metric": "batched jobs/s (256 jobs)", "value": round(len(sc) / T_batch, 2), "unit": "jobs/s",
         "n_gpus": 1, "steps": 3, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (256 seeded Cornell scenes, 2-40 spheres)",
         "config": {
-            "workload": "c4: 256 jobs, sides {64..512}, spp {4,16,64}, depth 6, RR from bounce 3",
+            "workload": "256 Cornell-box jobs with 2-40 subdivision-4 spheres (10,264-204,824 triangles), sides {64..512}, spp {4,16,64}, depth 6, RR from bounce 3",
             "triangles_min_max": [int(tri.min()), int(tri.max())], "paths_total": int(paths.sum()),
             "batch_s": round(T_batch, 4),
             "per_job_multisample_s": round(T_single, 4),
@@ -614,8 +614,8 @@ def workload_batch_c4(a) -> dict:
 
 
 WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m,
-             "render_c1": workload_render_c1, "render_c2": workload_render_c2,
-             "adjoint_c5": workload_adjoint_c5, "batch_c4": workload_batch_c4}
+             "render_cornell": workload_render_cornell, "render_spheres": workload_render_spheres,
+             "adjoint_cornell": workload_adjoint_cornell, "batch256": workload_batch256}
 
 
 def main() -> None:
@@ -629,7 +629,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
     ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
-    ap.add_argument("--batch-only", action="store_true", help="batch_c4: skip the per-job side")
+    ap.add_argument("--batch-only", action="store_true", help="batch256: skip the per-job side")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -646,9 +646,9 @@ def main() -> None:
         print(json.dumps({"copy_calibration_mib": a.copy_calibration, "launches": 5}))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the render core runs on a single GPU; DESIGN.md describes how it would shard)",
-              file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (the render core "
+              "runs on a single GPU: rays, pixels and jobs are independent, so N GPUs would each "
+              "run this benchmark's work on their own share)", file=sys.stderr)
         sys.exit(2)
     import torch
 

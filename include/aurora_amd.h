@@ -242,7 +242,7 @@ aur_status aur_frame_get_region(aur_context* ctx, uint32_t* left, uint32_t* righ
 /* 3x3 tent filter (1/16 1/8 1/16; 1/8 1/4 1/8; 1/16 1/8 1/16) of the region
  * in place, taps clamped to the region (cudaFilterPixels,
  * src/kernels/utility.cu:231-268; the reference clamps the row above to
- * region.bottom, F9, and never calls it).  Synchronous. */
+ * region.bottom, a bug, and never calls it).  Synchronous. */
 aur_status aur_frame_filter(aur_context* ctx);
 /* Loads host float RGBA (width*height*4) into the accumulation buffer: with
  * aur_frame_render_samples, resumes a progressive render from a saved
@@ -293,7 +293,7 @@ typedef struct aur_transform {
 } aur_transform;
 /* Uploads object-space triangles (layouts as aur_geometry_upload; count may
  * be odd: the last triangle is duplicated to an even count, triangle-wise,
- * where the reference copied floats, F9) and transforms them on the device:
+ * where the reference copied floats) and transforms them on the device:
  * triangle t takes the last transform, in array order, whose offset <= t
  * (the pad triangle included); positions by `world`, normals by the upper
  * 3x3 of `normal` (not renormalised); uvs and shader ids unchanged.

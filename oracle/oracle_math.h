@@ -11,7 +11,7 @@
 //   normalize uses rsqrtf on the GPU (src/util/math.h:210-211); rsqrtf's
 //   approximation is CUDA-specific, so the restatement is 1.0f/sqrtf(x),
 //   which is what the reference's host build uses (src/util/math.h:45-46).
-//   float min/max on the device are fminf/fmaxf (NaN-ignoring), see F11.
+//   float min/max on the device are fminf/fmaxf (NaN-ignoring), as here.
 #pragma once
 #include <cmath>
 #include <cstdint>

@@ -8,7 +8,7 @@
 //   cudaRaytraceKernel / raytrace  src/kernels/monteCarloRaytracer.cu:20-113
 //   Shader::getBSDF            src/kernels/lib/shader.cuh:8-27
 //   getBasisVectors            src/kernels/lib/common.cuh:53-66
-//   BSDF::f                    src/kernels/lib/bsdf.cuh:13-41 (Lambert == Phong, F3)
+//   BSDF::f                    src/kernels/lib/bsdf.cuh:13-41 (Phong evaluates as Lambert)
 //   estimate*Radiance          src/kernels/lib/radiance.cuh:8-80
 //   Light::sampleL / visiblePoint src/kernels/lib/light.cuh:24-60
 //   sampleRectangle            src/kernels/lib/sampling.cuh:148-152
@@ -761,7 +761,7 @@ int orc_adjoint(uint32_t count, const float* verts, const float* normals, const 
 // cudaFilterPixelsKernel restated (src/kernels/utility.cu:231-258): 3x3
 // tent over the inclusive region, taps clamped to the region, float4 sums in
 // the reference's order; the row above is clamped to region.top (the
-// reference uses region.bottom, F9).  in/out: float4 per pixel, w*h; out's
+// reference clamps it to region.bottom, a bug).  in/out: float4 per pixel, w*h; out's
 // pixels outside the region are copied from in.
 void orc_filter_pixels(uint32_t w, uint32_t h, const uint32_t* region, const float* in, float* out) {
   std::memcpy(out, in, size_t(w) * h * 4 * sizeof(float));

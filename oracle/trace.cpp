@@ -115,7 +115,7 @@ bool traverse(uint32_t count, const float* verts, Ray& ray, int32_t& id, float& 
       L = R;
       R = tmp;
     }
-    if (size + 2 > 32) return hit;  // the reference does not check (F9); trees here stay < 32 deep
+    if (size + 2 > 32) return hit;  // the reference does not check its 32-entry stack; trees here stay < 32 deep
     stack[size++] = TState{R, caxis, s.rx, s.ry};
     stack[size++] = TState{L, caxis, s.rx, s.ry};
   }

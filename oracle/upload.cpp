@@ -5,7 +5,7 @@
 // Restates, in order:
 //   Geometry::padToEven   src/data/geometry.cpp:164-176 (intent: the pad
 //                         triangle is a copy of the last one; the reference
-//                         indexes floats instead of triangles, F9)
+//                         indexes floats instead of triangles)
 //   cudaTransformKernel   src/kernels/utility.cu:96-125: triangle t takes
 //                         the last transform, in array order, with
 //                         offset <= t; no transform leaves it unwritten in

@@ -56,7 +56,7 @@ def test_nmh_deterministic_rebuild(gpu_ctx):
 
 
 def test_nmh_bitexact_spheres_1m(gpu_ctx, oracle):
-    s = scenes.spheres()  # config c2: 1,024,002 triangles
+    s = scenes.spheres()  # 200 spheres + ground: 1,024,002 triangles
     perm, stats = _build(gpu_ctx, s)
     assert s.count == 1_024_002
     assert np.array_equal(perm, oracle.nmh_build(s.vertices))

@@ -3,7 +3,7 @@ src/kernels/utility.cu:231-258) by closed forms: the 3x3 tent's impulse
 response, exact invariance of constant images (the weights are powers of two
 summing to 1), clamping at region edges (a one-pixel region is the identity,
 a one-row region the 1-D tent 1/4 1/2 1/4), the fixed clamp of the row above
-(the reference's F9 clamps it to region.bottom), and untouched pixels outside
+(the reference clamps it to region.bottom), and untouched pixels outside
 the region."""
 from __future__ import annotations
 
@@ -43,7 +43,7 @@ def test_constant_and_edges(oracle):
 
 def test_row_above_clamps_to_region_top(oracle):
     """At the region's top row the row above is the top row itself, not the
-    region's bottom row as in the reference (F9)."""
+    region's bottom row as in the reference."""
     w, h = 5, 5
     img = np.zeros((w * h, 4), np.float32)
     img[0 * w:1 * w] = 64.0  # bottom row bright

@@ -7,8 +7,8 @@
   the fallback axis.
 * Camera rays (src/kernels/utility.cu:18-77, src/util/camera.h:18-27) against
   the closed form.
-* Compat direct light (src/kernels/lib/radiance.cuh:30-80, F2: no 1/r^2, no
-  area, no pdf) z-tested per pixel against an FP64 quadrature of the
+* Compat direct light (src/kernels/lib/radiance.cuh:30-80; the reference's
+  area-light estimator has no 1/r^2, no area, no pdf) z-tested per pixel against an FP64 quadrature of the
   estimator's expectation.
 * PBR next-event estimation z-tested against Lambert's closed-form polygon
   irradiance (which is itself checked against FP64 quadrature).
