@@ -5,8 +5,10 @@ src/kernels/monteCarloRaytracer.cu:119-124, src/render/monteCarloRaytracer.cpp:1
 Here launch shapes are picked by measurement on the device at hand:
 
 * tune_trace: persistent vs flat traversal, block size, refill threshold and
-  resident waves per CU for aur_trace_closest/any, timed with HIP events on
-  the context's stream over the caller's own rays.
+  the LDS window of the traversal stack (a smaller window leaves room for
+  more resident waves, which pays for deep trees whose nodes miss the
+  caches) for aur_trace_closest/any, timed with HIP events on the context's
+  stream over the caller's own rays.
 * tune_batch: paths in flight per wavefront pass (queue sizes) and direct
   launches vs a replayed HIP graph for aur_batch_render, timed on a sample
   of the caller's jobs.
@@ -23,10 +25,10 @@ import os
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "tune.json")
 
 TRACE_CANDIDATES = (
-    [dict(persistent=0, block=b, refill=1, waves_per_cu=0, stack_lds_rows=0) for b in (64, 256)]
-    + [dict(persistent=1, block=b, refill=r, waves_per_cu=0, stack_lds_rows=0) for b in (128, 256)
-       for r in (16, 24, 32, 40, 48)]
-    + [dict(persistent=1, block=256, refill=32, waves_per_cu=0, stack_lds_rows=s) for s in (10, 14)]
+    [dict(persistent=0, block=64, refill=1, waves_per_cu=0, stack_lds_rows=0)]
+    + [dict(persistent=1, block=256, refill=r, waves_per_cu=0, stack_lds_rows=s)
+       for r in (16, 24, 32) for s in (0, 10, 12)]
+    + [dict(persistent=1, block=128, refill=r, waves_per_cu=0, stack_lds_rows=0) for r in (16, 24)]
 )
 
 

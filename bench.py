@@ -180,7 +180,7 @@ def workload_trace1m(a) -> dict:
 def workload_soup10m(a) -> dict:
     import torch
 
-    from aurora_amd import native, scenes
+    from aurora_amd import native, scenes, tune
 
     t0 = time.perf_counter()
     scene = scenes.soup(n=10_000_000)
@@ -189,16 +189,19 @@ def workload_soup10m(a) -> dict:
     ctx.upload_scene(scene)
     builds = [ctx.rebuild_nmh().build_ms for _ in range(3)]
     log(f"NMH build ms: {builds}")
-    out = {}
+    out, shapes = {}, {}
     for name, rays_np in (("incoherent", scenes.incoherent_rays(1 << 24, 0.0, 1000.0, seed=33)),
                           ("coherent", scenes.coherent_rays(scene.camera, 4096, 4096))):
         rays = torch.from_numpy(rays_np).to("cuda")
         hits = torch.empty((rays.shape[0], 4), dtype=torch.float32, device="cuda")
         n = rays.shape[0]
+        # launch shape per ray set by the autotuner (cached per device)
+        shapes[name], _ = tune.tune_trace(ctx, n, rays.data_ptr(), hits.data_ptr(), key=f"soup10m-{name}",
+                                          reps=2, force=a.tune)
         ms, _ = time_launches(ctx, lambda: ctx.trace_closest_ptr(n, rays.data_ptr(), hits.data_ptr()),
                               a.steps, a.warmup)
         out[name] = round(n / (ms * 1e-3) / 1e6, 1)
-        log(f"{name}: {ms:.3f} ms, {out[name]} Mrays/s")
+        log(f"{name}: {ms:.3f} ms, {out[name]} Mrays/s (launch shape {shapes[name]})")
     cpu = None
     if not a.no_cpu:
         from tests import oracle_ffi
@@ -219,7 +222,8 @@ def workload_soup10m(a) -> dict:
         "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded 10M-triangle soup)",
         "config": {"workload": "10M-triangle soup build + 2^24 incoherent + 4096^2 coherent rays",
                    "triangles": scene.count, "incoherent_mrays": out["incoherent"],
-                   "coherent_mrays": out["coherent"], "build_ms_all": [round(b, 3) for b in builds]},
+                   "coherent_mrays": out["coherent"], "trace_params": shapes,
+                   "build_ms_all": [round(b, 3) for b in builds]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": nb},
