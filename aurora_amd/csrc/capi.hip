@@ -76,9 +76,12 @@ aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* p) {
   if (p->max_paths > (1u << 26)) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: max_paths > 2^26");
   if (p->graph > 1) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: graph must be 0 or 1");
   if (p->refill > 64) return ctx->fail(AUR_ERR_INVALID_ARG, "render params: refill must be in 0..64");
+  if (p->stack_lds_rows > 64)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "render params: stack_lds_rows must be in 0..64");
   ctx->render_params.max_paths = p->max_paths ? p->max_paths : aur::kDefaultMaxPaths;
   ctx->render_params.graph = p->graph;
   ctx->render_params.refill = p->refill ? p->refill : aur::kDefaultRenderRefill;
+  ctx->render_params.stack_lds_rows = p->stack_lds_rows ? p->stack_lds_rows : aur::kDefaultRenderStackRows;
   return AUR_OK;
 }
 

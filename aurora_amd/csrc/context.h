@@ -54,6 +54,7 @@ struct DevBuf {
 // path state plus 48 B per next-event slot and path).
 constexpr uint32_t kDefaultMaxPaths = 1u << 24;
 constexpr uint32_t kDefaultRenderRefill = 16;
+constexpr uint32_t kDefaultRenderStackRows = 12;
 // Frame counters (u32): three 64-bit queue-count words of the wavefront
 // passes, then two u64 ray counters (camera + extension, shadow) at
 // kStatsOffset.
@@ -81,12 +82,14 @@ struct JobDev {
   uint32_t n_nodes, num_shaders, num_lights, nslots;
   uint32_t w, h, npix, spp, max_depth, rr_start, seed, paths;
   uint32_t rx0, ry0, rw;
+  float2 root;  // the tree's root slab (axis 0)
   aur_camera cam;
 };
 
 // A built scene kept for batches: tree-order arrays as the frame render uses.
 struct SceneStore {
   uint32_t count = 0, levels = 0, num_shaders = 0, num_lights = 0, nslots = 0;
+  float2 root_slab = make_float2(0.0f, 0.0f);
   DevBuf<NodeRec> nodes;
   DevBuf<float> normals;
   DevBuf<uint16_t> shader_ids;
@@ -107,6 +110,8 @@ struct FrameState {
   DevBuf<float4> sh_o, sh_d, sh_c;  // shadow rays: origin+tmax, dir, contribution
   DevBuf<int32_t> sh_occ;
   DevBuf<uint32_t> sh_queue;  // slots holding a shadow ray, appended by shading
+  DevBuf<float2> gstack;      // traversal stack rows beyond the LDS window
+  DevBuf<uint32_t> sh_job;    // batches: job of each shadow-queue entry
   // adjoint replay: path-vertex records, per-block partial gradients,
   // running gradient sums and the largest |dLoss/dI|
   DevBuf<int32_t> rec_mat;
@@ -133,7 +138,9 @@ struct aur_context {
   // limited residency: fastest shape measured at 1M triangles / 2^24 rays
   aur_trace_params trace_params{1, 256, 24, 0, 0};
   aur::DevBuf<float2> gstack;  // global rows of the persistent traversal stack
-  aur_render_params render_params{aur::kDefaultMaxPaths, 0, aur::kDefaultRenderRefill};
+  float2 root_slab = make_float2(0.0f, 0.0f);  // host copy of the built tree's root slab
+  aur_render_params render_params{aur::kDefaultMaxPaths, 0, aur::kDefaultRenderRefill,
+                                  aur::kDefaultRenderStackRows};
   std::atomic<uint32_t> cancel{0};  // set by aur_cancel, consumed by the render it stops
 
   // uploaded geometry (input order)

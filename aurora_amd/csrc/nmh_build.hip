@@ -721,6 +721,12 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // the root's own slab (cslab[0..1] of record n) for launches that start
+  // every ray at the root without reading it
+  float root[2] = {0.0f, 0.0f};
+  AUR_HIP(ctx, hipMemcpyAsync(root, &ctx->nodes.p[n].cslab[0], sizeof(root), hipMemcpyDeviceToHost, s));
+  AUR_HIP(ctx, hipStreamSynchronize(s));
+  ctx->root_slab = make_float2(root[0], root[1]);
   ctx->levels = (uint32_t)L;
   ctx->built = true;
   ctx->has_tb = false;

@@ -342,6 +342,10 @@ struct PbrArgs {
   unsigned long long end;
   uint32_t cap, nslots, max_depth;
   uint32_t refill;  // persistent traversal: refill once this many lanes are idle
+  // traversal stack: klds rows in LDS (all when gstack is null), the other
+  // gstack_rows per lane in gstack
+  float2* gstack;
+  uint32_t klds, gstack_rows;
   // rays of bounce k in queue order at [(k & 1) * cap + queue position];
   // hits of the current bounce in queue order
   float4 *ray_o, *ray_d;
@@ -350,6 +354,7 @@ struct PbrArgs {
   float4 *sh_o, *sh_d, *sh_c;
   int32_t* sh_occ;
   uint32_t* sh_queue;  // slot indices e = slot * cap + queue position with a shadow ray
+  uint32_t* sh_job;    // batches: the job of each shadow-queue entry
   // queue lengths as three 64-bit words used in turn, word k % 3 for bounce
   // k: counters[2w] = paths entering bounce k, counters[2w + 1] = shadow rays
   // shading appended at bounce k - 1 (shading appends to both with one
@@ -443,7 +448,9 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   const float py = ((float)y + fy) / (float)J.h - 0.5f;
   const V3 d = normalize(px * ld3(cam.right) * cam.tanfov[0] + py * ld3(cam.up) * cam.tanfov[1] +
                          ld3(cam.forward));
-  a.ray_o[q] = make_float4(cam.position[0], cam.position[1], cam.position[2], 0.0f);
+  // a batch carries each ray's job in origin.w (the trace needs no lookup)
+  a.ray_o[q] = make_float4(cam.position[0], cam.position[1], cam.position[2],
+                           kMulti ? __uint_as_float(jb) : 0.0f);
   a.ray_d[q] = make_float4(d.x, d.y, d.z, 0.0f);
   a.beta[q] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   a.radiance[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -457,12 +464,24 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
 // items [n_ext, n_ext + n_sh)); both counts in word w = k % 3.  parts bit 0:
 // extension rays, bit 1: shadow rays.  The two sets are independent, so one
 // launch overlaps their tails.
-template <bool kMulti>
+template <bool kMulti, bool kSplit>
 __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uint32_t parts,
                                                       uint32_t roff,
                                                       const uint32_t* __restrict__ qext,
                                                       const uint32_t* __restrict__ qsh) {
   extern __shared__ float2 stk_mem[];
+  const auto stack = [&]() {
+    if constexpr (kSplit) {
+      const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+      const uint32_t lanes = gridDim.x * blockDim.x;
+      return SplitStack{(lds_vf2*)(stk_mem + threadIdx.x), blockDim.x, (glb_vf2*)(a.gstack + gtid),
+                        __builtin_amdgcn_make_buffer_rsrc(a.gstack, (short)0,
+                                                          (int)(a.gstack_rows * lanes * 8u), 0x00020000),
+                        gtid * 8u, lanes, a.klds};
+    } else {
+      return lds_stack(stk_mem + threadIdx.x, blockDim.x);
+    }
+  }();
   const uint32_t n_ext = (parts & 1u) ? a.counters[2 * w] : 0u;
   const uint32_t n_sh = (parts & 2u) ? a.counters[2 * w + 1] : 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -474,29 +493,27 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
     atomicAdd(&a.stats[0], (unsigned long long)n_ext);
     atomicAdd(&a.stats[1], (unsigned long long)n_sh);
   }
-  persistent_trace<!kMulti>(
-      TreeRef{a.job.nodes, a.job.n_nodes}, (uint64_t)n_ext + n_sh, a.refill,
-      lds_stack(stk_mem + threadIdx.x, blockDim.x),
+  persistent_trace(
+      TreeRef{a.job.nodes, a.job.n_nodes, a.job.root}, (uint64_t)n_ext + n_sh, a.refill, stack,
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree, bool& any) {
         float4 o4, d4;
+        uint32_t jb = 0;
         if (i < n_ext) {
-          if constexpr (kMulti) {
-            const JobDev& J = a.jobs[a.path_job[qext[i]]];
-            tree = TreeRef{J.nodes, J.n_nodes};
-          }
           o4 = a.ray_o[roff + i];
           d4 = a.ray_d[roff + i];
+          jb = __float_as_uint(o4.w);
           o4.w = kInfinity;
           any = false;
         } else {
           const uint32_t e = a.sh_queue[i - n_ext];
-          if constexpr (kMulti) {
-            const JobDev& J = a.jobs[a.path_job[qsh[e % a.cap]]];
-            tree = TreeRef{J.nodes, J.n_nodes};
-          }
+          if constexpr (kMulti) jb = a.sh_job[i - n_ext];
           o4 = a.sh_o[e];
           d4 = a.sh_d[e];
           any = true;
+        }
+        if constexpr (kMulti) {
+          const JobDev& J = a.jobs[jb];
+          tree = TreeRef{J.nodes, J.n_nodes, J.root};
         }
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
@@ -657,16 +674,15 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32
   const uint32_t n_in = a.counters[2 * w];
   const uint32_t roff = (bounce & 1u) * a.cap, roff_next = a.cap - roff;
   ShadeOut out{false, 0ull, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
-  uint32_t p = 0;
+  uint32_t p = 0, jb = 0;
   if constexpr (kMulti) {
     // one round per distinct job among the wave's paths (almost always one:
     // paths are numbered job by job), each reading its job record at a
-    // wave-uniform address
+    // wave-uniform address; a ray carries its job in origin.w
     const bool act = j < n_in;
-    uint32_t jb = 0;
     if (act) {
       p = qin[j];
-      jb = a.path_job[p];
+      jb = __float_as_uint(a.ray_o[roff + j].w);
     }
     uint64_t todo = __ballot(act);
     while (todo) {
@@ -698,6 +714,7 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32
   if (out.survive) {
     const uint32_t pos = (uint32_t)old + (uint32_t)__popcll(ms & below);
     qout[pos] = p;
+    if constexpr (kMulti) out.o.w = __uint_as_float(jb);  // the ray's job, for the trace
     a.ray_o[roff_next + pos] = out.o;
     a.ray_d[roff_next + pos] = out.d;
   }
@@ -705,7 +722,11 @@ __global__ __launch_bounds__(128) void k_pbr_shade(PbrArgs a, uint32_t w, uint32
   for (uint32_t sl = 0; sl < a.nslots; ++sl) {
     const bool mine = (out.shadows >> sl) & 1ull;
     const uint64_t m = __ballot(mine);
-    if (mine) a.sh_queue[qbase + (uint32_t)__popcll(m & below)] = sl * a.cap + j;
+    if (mine) {
+      const uint32_t qpos = qbase + (uint32_t)__popcll(m & below);
+      a.sh_queue[qpos] = sl * a.cap + j;
+      if constexpr (kMulti) a.sh_job[qpos] = jb;
+    }
     qbase += (uint32_t)__popcll(m);
   }
 }
@@ -993,7 +1014,13 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
   // bounce b: shade, then one launch tracing bounce b + 1's extension rays
   // with bounce b's shadow rays, then bounce b's next-event resolve
   const uint32_t sh = a.nslots ? 2u : 0u;
-  k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, 0u, 1u, 0u, fr.queue[0].p, fr.queue[1].p);
+  const auto trace = [&](uint32_t w, uint32_t parts, uint32_t roff, const uint32_t* qe, const uint32_t* qs) {
+    if (a.gstack)
+      k_pbr_trace<kMulti, true><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff, qe, qs);
+    else
+      k_pbr_trace<kMulti, false><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff, qe, qs);
+  };
+  trace(0u, 1u, 0u, fr.queue[0].p, fr.queue[1].p);
   for (uint32_t b = 0; b < a.max_depth; ++b) {
     const uint32_t w = b % 3u, wn = (b + 1u) % 3u;
     uint32_t* qin = fr.queue[b & 1u].p;
@@ -1004,7 +1031,7 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
       k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout, a.jobs);
     const uint32_t parts = (b + 1u < a.max_depth ? 1u : 0u) | sh;
     if (parts)
-      k_pbr_trace<kMulti><<<grid_trace, kBlock, lds, s>>>(a, wn, parts, ((b + 1u) & 1u) * cap, qout, qin);
+      trace(wn, parts, ((b + 1u) & 1u) * cap, qout, qin);
     if (a.nslots) {
       if (adj)
         k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, w, b, qin);
@@ -1041,6 +1068,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   if (kMulti) {
     AUR_HIP(ctx, fr.jobs.reserve(run.njobs));
     AUR_HIP(ctx, fr.path_job.reserve(cap));
+    AUR_HIP(ctx, fr.sh_job.reserve(std::max<size_t>(1, (size_t)cap * run.nslots)));
     AUR_HIP(ctx, hipMemcpyAsync(fr.jobs.p, run.jobs, sizeof(JobDev) * run.njobs, hipMemcpyHostToDevice, s));
     AUR_HIP(ctx, hipStreamSynchronize(s));  // run.jobs is pageable host memory
     a.jobs = fr.jobs.p;
@@ -1064,6 +1092,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   a.sh_c = fr.sh_c.p;
   a.sh_occ = fr.sh_occ.p;
   a.sh_queue = fr.sh_queue.p;
+  a.sh_job = kMulti ? fr.sh_job.p : nullptr;
   a.counters = fr.counters.p;
   a.stats = stats;
   if (adj) {
@@ -1071,9 +1100,26 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     a.rec_rr = fr.rec_rr.p;
     a.rec_w = fr.rec_w.p;
   }
-  const size_t lds = (size_t)run.levels * kBlock * sizeof(float2);
+  // traversal stack: at most levels - 1 rows per lane; the first `rows` in
+  // LDS, the rest (with a window) in global memory
+  const uint32_t maxrows = run.levels > 1 ? run.levels - 1 : 1;
+  const uint32_t want = ctx->render_params.stack_lds_rows;
+  const uint32_t rows = want >= maxrows ? maxrows : want;
+  const size_t lds = (size_t)rows * kBlock * sizeof(float2);
   const uint32_t grid_trace =
-      resident_grid(ctx, (const void*)k_pbr_trace<kMulti>, lds, (uint64_t)cap * (1u + run.nslots));
+      resident_grid(ctx, rows < maxrows ? (const void*)k_pbr_trace<kMulti, true> : (const void*)k_pbr_trace<kMulti, false>,
+                    lds, (uint64_t)cap * (1u + run.nslots));
+  a.gstack = nullptr;
+  a.klds = rows;
+  a.gstack_rows = 0;
+  if (rows < maxrows) {
+    const size_t entries = (size_t)(maxrows - rows) * grid_trace * kBlock;
+    if (entries * sizeof(float2) >= (1ull << 31))
+      return ctx->fail(AUR_ERR_INVALID_ARG, "render: global stack rows exceed 2 GiB");
+    AUR_HIP(ctx, fr.gstack.reserve(entries));
+    a.gstack = fr.gstack.p;
+    a.gstack_rows = maxrows - rows;
+  }
   uint32_t sweep_grid = 0;
   if (adj) {
     // a few resident blocks per CU; each keeps LDS accumulators for all parameters
@@ -1162,6 +1208,7 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   J.num_textures = ctx->num_textures;
   J.path_start = 0;
   J.n_nodes = ctx->count / 2;
+  J.root = ctx->root_slab;
   J.num_shaders = ctx->num_shaders;
   J.num_lights = ctx->num_lights;
   J.nslots = ctx->num_slots;

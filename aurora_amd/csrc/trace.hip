@@ -44,7 +44,7 @@ __device__ __forceinline__ void trace_rays(TreeRef tree, const aur_ray* __restri
   uint64_t next = wave * chunk;
   const uint64_t end = next + chunk < n ? next + chunk : n;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const float2 root = root_slab(tree.nodes, tree.n_nodes);
+  const float2 root = root_slab(tree.nodes, tree.n_nodes);  // once per wave
   Trav s;
   bool active = false;
   uint64_t item = 0;

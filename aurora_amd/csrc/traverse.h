@@ -336,10 +336,12 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   return !found;
 }
 
-// The tree a lane traverses: node records and their count.
+// The tree a lane traverses: node records, their count and the root's own
+// slab (cslab[0..1] of the record after the last node).
 struct TreeRef {
   const NodeRec* nodes;
   uint32_t n_nodes;
+  float2 root;
 };
 
 // Persistent-wave traversal of a work range [0, n) of closest-hit and
@@ -349,16 +351,14 @@ struct TreeRef {
 // fetch(i, o, d, tmax, tree, any) loads item i, may replace the lane's tree
 // (initially tree0; batches of scenes), sets any for an any-hit query and
 // returns false if the item needs no traversal; store(i, s) receives the
-// finished state.  kOneTree: fetch never writes `tree`, so the tree stays
-// wave-uniform and its root slab is read once per wave.  (Splitting a refill
-// over two iterations, so that the item loads overlap the active lanes' node
-// step, measured slower: the new lanes lose a step.)
-template <bool kOneTree, class Stk, class Fetch, class Store>
+// finished state.  A fetch that never writes `tree` keeps the tree
+// wave-uniform.  (Splitting a refill over two iterations, so that the item
+// loads overlap the active lanes' node step, measured slower: the new lanes
+// lose a step.)
+template <class Stk, class Fetch, class Store>
 __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint32_t refill,
                                                  const Stk& stk, Fetch fetch, Store store) {
   TreeRef tree = tree0;
-  float2 root = make_float2(0.0f, 0.0f);
-  if constexpr (kOneTree) root = root_slab(tree0.nodes, tree0.n_nodes);
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t chunk = (n + waves - 1) / waves;
   const uint32_t lane = threadIdx.x & 63u;
@@ -382,10 +382,7 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint
             bool any = false;
             if (fetch(i, o, d, tmax, tree, any)) {
               item = i;
-              if constexpr (kOneTree)
-                active = trav_init_root<false>(s, root, o, d, tmax, nullptr);
-              else
-                active = trav_init<false>(s, tree.nodes, tree.n_nodes, o, d, tmax, nullptr);
+              active = trav_init_root<false>(s, tree.root, o, d, tmax, nullptr);
               s.any = any;
               if (!active) store(i, s);
             }

@@ -22,7 +22,7 @@ AUR_ERR_CANCELLED = 4
 STATUS_NAMES = {0: "OK", 1: "OOM", 2: "INVALID_ARG", 3: "DEVICE", 4: "CANCELLED", 5: "UNSUPPORTED",
                 6: "STATE"}
 MODE_COMPAT, MODE_PBR = 0, 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("reserved", "<f4")])
 HIT_DTYPE = np.dtype([("id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -57,7 +57,7 @@ class TraceParams(ctypes.Structure):
 
 class RenderParams(ctypes.Structure):
     _fields_ = [("max_paths", ctypes.c_uint32), ("graph", ctypes.c_uint32),
-                ("refill", ctypes.c_uint32)]
+                ("refill", ctypes.c_uint32), ("stack_lds_rows", ctypes.c_uint32)]
 
 
 # aur_job (include/aurora_amd.h): 8 x u32, u64 out_offset, aur_camera, padded to 8 B
@@ -294,14 +294,16 @@ class Context:
         self._check(self._lib.aur_get_trace_params(self._h, ctypes.byref(p)), "get_trace_params")
         return {k: getattr(p, k) for k, _ in TraceParams._fields_}
 
-    def set_render_params(self, max_paths: int = 0, graph: int = 0, refill: int = 0) -> None:
-        p = RenderParams(max_paths, graph, refill)
+    def set_render_params(self, max_paths: int = 0, graph: int = 0, refill: int = 0,
+                          stack_lds_rows: int = 0) -> None:
+        p = RenderParams(max_paths, graph, refill, stack_lds_rows)
         self._check(self._lib.aur_set_render_params(self._h, ctypes.byref(p)), "set_render_params")
 
     def get_render_params(self) -> dict:
         p = RenderParams()
         self._check(self._lib.aur_get_render_params(self._h, ctypes.byref(p)), "get_render_params")
-        return {"max_paths": p.max_paths, "graph": p.graph, "refill": p.refill}
+        return {"max_paths": p.max_paths, "graph": p.graph, "refill": p.refill,
+                "stack_lds_rows": p.stack_lds_rows}
 
     def trace_closest_stats_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> tuple[int, int]:
         p, q = ctypes.c_uint64(), ctypes.c_uint64()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AUR_ABI_VERSION 2u
+#define AUR_ABI_VERSION 3u
 
 typedef int32_t aur_status;
 enum {
@@ -214,12 +214,17 @@ aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
  * graph and replays it for every pass; 0 launches the kernels directly.
  * refill: the render's persistent traversal launches refill idle lanes once
  * this many (1..64) are idle; 0 selects the default (16).
+ * stack_lds_rows: how many rows of each lane's traversal stack live in LDS
+ * in those launches; deeper rows go to global memory, leaving LDS for more
+ * resident waves.  0 selects the default (12); a value of at least the
+ * tree's levels - 1 keeps every row in LDS.
  * Images do not depend on any of them: samples are accumulated in sample
  * order. */
 typedef struct aur_render_params {
   uint32_t max_paths;
   uint32_t graph;
   uint32_t refill;
+  uint32_t stack_lds_rows;
 } aur_render_params;
 aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* params);
 aur_status aur_get_render_params(aur_context* ctx, aur_render_params* params);

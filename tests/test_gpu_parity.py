@@ -228,10 +228,11 @@ def test_render_progressive_equals_full(gpu_ctx):
     assert np.array_equal(full, split)
 
 
-@pytest.mark.parametrize("max_paths,refill", [(1, 0), (3 * 64 * 64, 1), (0, 0), (0, 40)])
-def test_render_pass_batching_invariant(gpu_ctx, max_paths, refill):
-    """Samples per pass (paths in flight) and the traversal refill threshold
-    must not change a single bit."""
+@pytest.mark.parametrize("max_paths,refill,rows", [(1, 0, 64), (3 * 64 * 64, 1, 0), (0, 0, 0),
+                                                   (0, 40, 64), (0, 0, 3), (5000, 16, 8)])
+def test_render_pass_batching_invariant(gpu_ctx, max_paths, refill, rows):
+    """Samples per pass (paths in flight), the traversal refill threshold and
+    the LDS window of its stack must not change a single bit."""
     s = scenes.cornell(width=64, height=64)
     _build(gpu_ctx, s)
     gpu_ctx.frame_create(64, 64, 8)
@@ -240,7 +241,7 @@ def test_render_pass_batching_invariant(gpu_ctx, max_paths, refill):
         gpu_ctx.set_render_params(1)  # one sample per pass
         gpu_ctx.frame_render(s.camera, o)
         one = gpu_ctx.frame_read()
-        gpu_ctx.set_render_params(max_paths, 0, refill)
+        gpu_ctx.set_render_params(max_paths, 0, refill, rows)
         gpu_ctx.frame_render(s.camera, o)
         many = gpu_ctx.frame_read()
     finally:
