@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -25,6 +26,18 @@ bool finite_all(const float* p, size_t n) {
 }
 
 }  // namespace
+
+namespace aur {
+namespace {
+std::mutex g_hook_mutex;
+aur_alloc_hook g_hook{nullptr, nullptr, nullptr};
+}  // namespace
+
+aur_alloc_hook current_alloc_hook() {
+  std::lock_guard<std::mutex> lock(g_hook_mutex);
+  return g_hook;
+}
+}  // namespace aur
 
 extern "C" {
 
@@ -109,6 +122,13 @@ void* aur_stream(aur_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; 
 aur_status aur_synchronize(aur_context* ctx) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return AUR_OK;
+}
+
+aur_status aur_set_alloc_hook(const aur_alloc_hook* hook) {
+  if (hook && (!hook->alloc || !hook->free)) return AUR_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(aur::g_hook_mutex);
+  aur::g_hook = hook ? *hook : aur_alloc_hook{nullptr, nullptr, nullptr};
   return AUR_OK;
 }
 

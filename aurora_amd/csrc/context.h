@@ -15,11 +15,16 @@
 
 namespace aur {
 
-// Owning device allocation (hipMalloc / hipFree).
+// The device allocator in force (aur_set_alloc_hook); alloc == nullptr means
+// hipMalloc / hipFree.
+aur_alloc_hook current_alloc_hook();
+
+// Owning device allocation, freed by the allocator that made it.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  aur_alloc_hook by{nullptr, nullptr, nullptr};
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -27,9 +32,17 @@ struct DevBuf {
   void swap(DevBuf& o) {
     std::swap(p, o.p);
     std::swap(n, o.n);
+    std::swap(by, o.by);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (by.alloc) {
+        (void)hipDeviceSynchronize();  // as hipFree does: no queued work still uses p
+        by.free(p, by.user);
+      } else {
+        (void)hipFree(p);
+      }
+    }
     p = nullptr;
     n = 0;
   }
@@ -38,20 +51,27 @@ struct DevBuf {
     if (count <= n && p) return hipSuccess;
     release();
     if (count == 0) return hipSuccess;
+    const aur_alloc_hook h = current_alloc_hook();
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return e;
+    if (h.alloc) {
+      q = h.alloc(count * sizeof(T), h.user);
+      if (!q) return hipErrorOutOfMemory;
+    } else {
+      hipError_t e = hipMalloc(&q, count * sizeof(T));
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+      }
     }
     p = static_cast<T*>(q);
     n = count;
+    by = h;
     return hipSuccess;
   }
 };
 
-// Default paths in flight per PBR wavefront pass (2^21 paths: about 0.2 GB of
-// path state plus 48 B per next-event slot and path).
+// Default paths in flight per PBR wavefront pass (2^24 paths: about 2 GB of
+// path state plus 56 B per next-event slot and path).
 constexpr uint32_t kDefaultMaxPaths = 1u << 24;
 constexpr uint32_t kDefaultRenderRefill = 16;
 constexpr uint32_t kDefaultRenderStackRows = 12;

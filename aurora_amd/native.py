@@ -72,7 +72,17 @@ JOB_DTYPE = np.dtype({
 # aur_transform (include/aurora_amd.h): the reference's Transform, 136 B
 TRANSFORM_DTYPE = scenes.TRANSFORM_DTYPE
 
+# aur_alloc_hook (include/aurora_amd.h)
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class AllocHook(ctypes.Structure):
+    _fields_ = [("alloc", ALLOC_FN), ("free", FREE_FN), ("user", ctypes.c_void_p)]
+
+
 _lib = None
+_hooks = []  # every installed hook: callbacks must outlive the buffers they made
 
 
 def header_symbols() -> list[str]:
@@ -99,6 +109,7 @@ def load():
         "aur_stream": ([P], P),
         "aur_synchronize": ([P], i32),
         "aur_cancel": ([P], i32),
+        "aur_set_alloc_hook": ([ctypes.POINTER(AllocHook)], i32),
         "aur_geometry_upload": ([P, u32, P, P, P, P], i32),
         "aur_shaders_upload": ([P, u32, P], i32),
         "aur_lights_upload": ([P, u32, P], i32),
@@ -145,6 +156,41 @@ def load():
         fn.restype = res
     _lib = L
     return L
+
+
+def set_alloc_hook(alloc=None, free=None) -> None:
+    """Routes every later device allocation of the library through
+    alloc(nbytes) -> int device pointer (0 when out of memory) and
+    free(ptr); with no arguments restores hipMalloc / hipFree.  Buffers are
+    freed by the allocator that made them."""
+    lib = load()
+    if alloc is None:
+        st = lib.aur_set_alloc_hook(None)
+        if st != AUR_OK:
+            raise AuroraError(st, "aur_set_alloc_hook")
+        return
+    h = AllocHook(ALLOC_FN(lambda n, user: alloc(int(n)) or None), FREE_FN(lambda p, user: free(int(p))), None)
+    st = lib.aur_set_alloc_hook(ctypes.byref(h))
+    if st != AUR_OK:
+        raise AuroraError(st, "aur_set_alloc_hook")
+    _hooks.append(h)
+
+
+def torch_alloc_hook():
+    """(alloc, free) over PyTorch's caching allocator on the current device,
+    for set_alloc_hook: the library's buffers then come from torch's pool."""
+    import torch
+
+    def alloc(n: int) -> int:
+        try:
+            return int(torch.cuda.caching_allocator_alloc(n, torch.cuda.current_device()))
+        except RuntimeError:
+            return 0
+
+    def free(p: int) -> None:
+        torch.cuda.caching_allocator_delete(p)
+
+    return alloc, free
 
 
 def _ptr(a: np.ndarray | None):

@@ -121,6 +121,22 @@ aur_status aur_synchronize(aur_context* ctx);
  * reference's stop-on-interrupt of the IPR loop (src/core/engine.cpp:258-317,
  * m_pause / MComputation::isInterruptRequested). */
 aur_status aur_cancel(aur_context* ctx);
+/* Device memory allocator for every buffer the library allocates from now
+ * on, in any context (no reference counterpart: the reference calls
+ * cudaMalloc / cudaFree directly, src/util/array.h:22-32).  alloc returns
+ * device memory of the current device, aligned to at least 256 bytes, or
+ * NULL when out of memory (the call then fails with AUR_ERR_OOM); free
+ * receives exactly the pointers alloc returned, after the library has
+ * synchronised the device (as hipFree does), so no queued work still uses
+ * the memory.  A buffer is freed by the
+ * allocator that made it, so the hook may change while contexts live.  NULL
+ * restores hipMalloc / hipFree.  Set it while no other library call runs. */
+typedef struct aur_alloc_hook {
+  void* (*alloc)(size_t bytes, void* user);
+  void (*free)(void* ptr, void* user);
+  void* user;
+} aur_alloc_hook;
+aur_status aur_set_alloc_hook(const aur_alloc_hook* hook);
 
 /* ---- scene upload (replaces Geometry::resize/copyToDevice and
  * Scene::updateShaders/updateLights uploads: src/data/geometry.cpp:88-151,
