@@ -87,9 +87,13 @@ __global__ void k_bounds(const float* __restrict__ v, uint32_t N, float* __restr
   }
 }
 
-__global__ void k_iota(uint32_t* __restrict__ W, uint32_t N) {
+// Identity order, and (when K is given) the first level's axis-0 sort keys.
+__global__ void k_iota(uint32_t* __restrict__ W, uint32_t N, const float* __restrict__ key0,
+                       uint32_t* __restrict__ K) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < N) W[i] = i;
+  if (i >= N) return;
+  W[i] = i;
+  if (K) K[i] = fkey(key0[i]);
 }
 
 // Indices read back from device arrays are clamped to [0, N) before they
@@ -100,12 +104,6 @@ __device__ __forceinline__ uint32_t clampi(uint32_t i, uint32_t N) { return i < 
 __global__ void k_fill0(uint32_t* __restrict__ p, uint32_t N) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N) p[i] = 0;
-}
-
-__global__ void k_gather_keys(const uint32_t* __restrict__ W, uint32_t m, uint32_t N,
-                              const float* __restrict__ keys, uint32_t* __restrict__ K) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) K[i] = fkey(keys[clampi(W[i], N)]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restrict__ tiles,
@@ -132,6 +130,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_chunk_sums(const ChunkDesc* 
   const ChunkDesc c = chunks[blockIdx.x];
   const uint32_t d = threadIdx.x;
   uint32_t sum = 0;
+#pragma unroll 8
   for (uint32_t t = c.tb; t < c.te; ++t) sum += H[(size_t)t * 256 + d];
   S[(size_t)blockIdx.x * 256 + d] = sum;
 }
@@ -143,6 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
   const GroupDesc g = groups[blockIdx.x];
   const uint32_t d = threadIdx.x;
   uint32_t total = 0;
+#pragma unroll 8
   for (uint32_t c = g.cb; c < g.ce; ++c) total += S[(size_t)c * 256 + d];
   s[d] = total;
   __syncthreads();
@@ -153,6 +153,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
     __syncthreads();
   }
   uint32_t run = g.off + s[d] - total;
+#pragma unroll 8
   for (uint32_t c = g.cb; c < g.ce; ++c) {
     B[(size_t)c * 256 + d] = run;
     run += S[(size_t)c * 256 + d];
@@ -166,24 +167,44 @@ __global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const ChunkDes
   const ChunkDesc c = chunks[blockIdx.x];
   const uint32_t d = threadIdx.x;
   uint32_t run = B[(size_t)blockIdx.x * 256 + d];
+#pragma unroll 8
   for (uint32_t t = c.tb; t < c.te; ++t) {
     O[(size_t)t * 256 + d] = run;
     run += H[(size_t)t * 256 + d];
   }
 }
 
-// Stable scatter: elements of a tile are ranked in order within their digit.
-// Rounds of 256 elements; inside a wave, lanes with the same digit are found
-// with 8 ballots; per-wave digit counts go through LDS in wave order.
+// Stable scatter, staged through LDS.  Elements of a tile are ranked in order
+// within their digit: rounds of 256 elements; inside a wave, lanes with the
+// same digit are found with 8 ballots; per-wave digit counts go through LDS in
+// wave order.  Each element lands in LDS at its digit-sorted tile position
+// (the tile's own exclusive digit offsets, from its histogram), and the tile
+// is then written out in that order: a digit's run is one contiguous store.
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
-    const uint32_t* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ O, uint32_t m,
-    uint32_t* __restrict__ Kout, uint32_t* __restrict__ Vout) {
-  __shared__ uint32_t base[256];
+    const uint32_t* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ H,
+    const uint32_t* __restrict__ O, uint32_t m, uint32_t* __restrict__ Kout,
+    uint32_t* __restrict__ Vout) {
+  __shared__ uint32_t sk[kTile], sv[kTile];
+  __shared__ uint32_t base[256];   // next free tile position per digit
+  __shared__ uint32_t delta[256];  // output position - tile position, per digit
   __shared__ uint32_t wcnt[kThreads / 64][256];
   const TileDesc t = tiles[blockIdx.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  base[tid] = O[(size_t)blockIdx.x * 256 + tid];
+  const uint32_t cnt = H[(size_t)blockIdx.x * 256 + tid];
+  // exclusive scan of the tile histogram: digit tid's first tile position
+  base[tid] = cnt;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = tid >= off ? base[tid - off] : 0u;
+    __syncthreads();
+    base[tid] += v;
+    __syncthreads();
+  }
+  const uint32_t first = base[tid] - cnt;
+  __syncthreads();
+  base[tid] = first;
+  delta[tid] = O[(size_t)blockIdx.x * 256 + tid] - first;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t r0 = 0; r0 < t.count; r0 += kThreads) {
 #pragma unroll
@@ -207,9 +228,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     if (valid) {
       uint32_t pos = base[dg] + rank;
       for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][dg];
-      if (pos < m) {
-        Kout[pos] = k;
-        Vout[pos] = v;
+      if (pos < kTile) {
+        sk[pos] = k;
+        sv[pos] = v;
       }
     }
     __syncthreads();
@@ -218,6 +239,15 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     for (int w = 0; w < kThreads / 64; ++w) add += wcnt[w][tid];
     base[tid] += add;
     __syncthreads();
+  }
+  const uint32_t n_out = min(t.count, kTile);
+  for (uint32_t j = tid; j < n_out; j += kThreads) {
+    const uint32_t k = sk[j];
+    const uint32_t pos = delta[(k >> shift) & 255u] + j;
+    if (pos < m) {
+      Kout[pos] = k;
+      Vout[pos] = sv[j];
+    }
   }
 }
 
@@ -264,32 +294,46 @@ __global__ __launch_bounds__(kThreads) void k_argmax_partial(const TileDesc* __r
   }
 }
 
-__global__ void k_argmax_combine(const ChildDesc* __restrict__ children, uint32_t nchildren,
-                                 const uint2* __restrict__ partial, uint32_t* __restrict__ argpos) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nchildren) return;
+// One wave per child: (max key, lowest position) over the child's tiles.
+__global__ __launch_bounds__(kThreads) void k_argmax_combine(const ChildDesc* __restrict__ children,
+                                                             uint32_t nchildren,
+                                                             const uint2* __restrict__ partial,
+                                                             uint32_t* __restrict__ argpos) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (c >= nchildren) return;  // uniform per wave
   const ChildDesc d = children[c];
   uint32_t bk = 0, bp = 0xFFFFFFFFu;
-  for (uint32_t t = d.tb; t < d.te; ++t) {
+  for (uint32_t t = d.tb + lane; t < d.te; t += 64) {
     const uint2 p = partial[t];
     if (p.x > bk || (p.x == bk && p.y < bp)) {
       bk = p.x;
       bp = p.y;
     }
   }
-  argpos[c] = bp;
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t ok = __shfl_xor(bk, off), op = __shfl_xor(bp, off);
+    if (ok > bk || (ok == bk && op < bp)) {
+      bk = ok;
+      bp = op;
+    }
+  }
+  if (lane == 0) argpos[c] = bp;
 }
 
-// Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders.
+// Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders;
+// with Knext, also the next level's sort keys of the compacted remainders.
 __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ tiles,
                                                    const ChildDesc* __restrict__ children,
                                                    const uint32_t* __restrict__ argpos,
                                                    const uint32_t* __restrict__ W,
                                                    uint32_t* __restrict__ Wnext, uint32_t first,
                                                    int level, int L, const float* __restrict__ bmin,
-                                                   uint32_t N, uint32_t* __restrict__ P) {
+                                                   uint32_t N, uint32_t* __restrict__ P,
+                                                   uint32_t* __restrict__ Knext) {
   const TileDesc t = tiles[blockIdx.x];
   const ChildDesc c = children[t.child];
+  const float* knext = bmin + (size_t)((level + 1) % 3) * N;
   uint32_t am = argpos[t.child];
   if (am <= c.off || am >= c.off + c.size) am = c.off + 1;  // cannot happen for sizes >= 2
   for (uint32_t r = threadIdx.x; r < t.count; r += kThreads) {
@@ -298,7 +342,9 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
       write_pair(first + t.child, W[c.off], W[am], level, L, bmin, N, P);
     } else if (i >= c.off + 2) {
       const uint32_t val = (i == am) ? W[c.off + 1] : W[i];
-      Wnext[i - 2 * (t.child + 1)] = val;
+      const uint32_t dst = i - 2 * (t.child + 1);
+      Wnext[dst] = val;
+      if (Knext) Knext[dst] = fkey(knext[clampi(val, N)]);
     }
   }
 }
@@ -343,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict
     for (uint32_t k = 2; k <= M; k <<= 1) {
       for (uint32_t j = k >> 1; j > 0; j >>= 1) {
         for (uint32_t t = tid; t < M / 2; t += kThreads) {
-          const uint32_t i = 2 * j * (t / j) + (t % j);
+          const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));  // j is a power of two
           const uint32_t ixj = i + j;
           const bool up = (i & k) == 0;
           const unsigned long long x = K[i], y = K[ixj];
@@ -669,17 +715,19 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, hipEventRecord(e0, s));
 
   k_bounds<<<blocks(N, 256), 256, 0, s>>>(ctx->in_vertices.p, N, bmin.p, bmax.p);
-  k_iota<<<blocks(N, 256), 256, 0, s>>>(Wa.p, N);
+  // level 0's keys come with the identity order; every later level's keys
+  // are gathered by the previous level's emit pass
+  k_iota<<<blocks(N, 256), 256, 0, s>>>(Wa.p, N, bmin.p, plan.empty() ? nullptr : Ka.p);
   k_fill0<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N);
   uint32_t* Wcur = Wa.p;
   uint32_t* Walt = Wb.p;
-  for (const LevelPlan& lp : plan) {
+  for (size_t li = 0; li < plan.size(); ++li) {
+    const LevelPlan& lp = plan[li];
     const int a = lp.level % 3;
     const TileDesc* tl = dtiles.p + lp.tile_base;
     const ChildDesc* ch = dchildren.p + lp.child_base;
     const GroupDesc* gr = dgroups.p + lp.group_base;
     const ChunkDesc* ck = dchunks.p + lp.chunk_base;
-    k_gather_keys<<<blocks(lp.m, 256), 256, 0, s>>>(Wcur, lp.m, N, bmin.p + (size_t)a * N, Ka.p);
     uint32_t *kin = Ka.p, *kout = Kb.p, *vin = Wcur, *vout = Walt;
     for (int pass = 0; pass < 4; ++pass) {
       const uint32_t shift = 8u * pass;
@@ -687,15 +735,17 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
       k_radix_chunk_sums<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CS.p);
       k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, CS.p, CB.p);
       k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CB.p, O.p);
-      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, O.p, lp.m, kout, vout);
+      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, H.p, O.p, lp.m, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);
     }
-    // after 4 passes the sorted data is back in (Ka, Wcur)
+    // after 4 passes the sorted data is back in (Ka, Wcur); Ka is free again
     k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Wcur, N, bmax.p + (size_t)a * N, partial.p);
-    k_argmax_combine<<<blocks(lp.nchildren, 256), 256, 0, s>>>(ch, lp.nchildren, partial.p, argpos.p);
+    k_argmax_combine<<<blocks(lp.nchildren, kThreads / 64), kThreads, 0, s>>>(ch, lp.nchildren,
+                                                                              partial.p, argpos.p);
     k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Wcur, Walt, lp.first, lp.level, L,
-                                          bmin.p, N, ctx->perm.p);
+                                          bmin.p, N, ctx->perm.p,
+                                          li + 1 < plan.size() ? Ka.p : nullptr);
     std::swap(Wcur, Walt);
   }
   AUR_LAUNCH_CHECK(ctx, "nmh level kernels");
