@@ -56,22 +56,33 @@ struct SubGroup {
   uint32_t off, size, first, count;
 };
 
-__device__ inline void write_pair(uint32_t h, uint32_t x, uint32_t y, int level, int L,
-                                  const float* __restrict__ bmin, uint32_t N,
-                                  uint32_t* __restrict__ P) {
-  bool y_first = false;
-  const int lo = max(level + 1, L - 3);
-  for (int lev = L - 1; lev >= lo; --lev) {
-    const float* km = bmin + (size_t)(lev % 3) * N;
-    const float kx = km[x], ky = km[y];
-    if (ky < kx) {
-      y_first = true;
-      break;
+// kx, ky: the min keys of x and y on the axis of the last level (L - 1),
+// loaded by the caller; level <= L - 2.
+__device__ inline void write_pair_k(uint32_t h, uint32_t x, uint32_t y, float kx, float ky,
+                                    int level, int L, const float* __restrict__ bmin, uint32_t N,
+                                    uint32_t* __restrict__ P) {
+  bool y_first = ky < kx;
+  if (!y_first && !(kx < ky)) {
+    const int lo = max(level + 1, L - 3);
+    for (int lev = L - 2; lev >= lo; --lev) {
+      const float* km = bmin + (size_t)(lev % 3) * N;
+      const float ax = km[x], ay = km[y];
+      if (ay < ax) {
+        y_first = true;
+        break;
+      }
+      if (ax < ay) break;
     }
-    if (kx < ky) break;
   }
   P[2 * (size_t)h] = y_first ? y : x;
   P[2 * (size_t)h + 1] = y_first ? x : y;
+}
+
+__device__ inline void write_pair(uint32_t h, uint32_t x, uint32_t y, int level, int L,
+                                  const float* __restrict__ bmin, uint32_t N,
+                                  uint32_t* __restrict__ P) {
+  const float* km = bmin + (size_t)((L - 1) % 3) * N;
+  write_pair_k(h, x, y, km[x], km[y], level, L, bmin, N, P);
 }
 
 // computeTriangleBounds (rebuilder.cu:72-91), SoA [axis][triangle].
@@ -350,19 +361,23 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
 }
 
 // One workgroup per group: all remaining levels of the group's subtree(s).
-__global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict__ groups,
-                                                      const uint32_t* __restrict__ W,
-                                                      const float* __restrict__ bmin,
-                                                      const float* __restrict__ bmax, uint32_t N,
-                                                      uint32_t n, int L, int level0,
-                                                      uint32_t* __restrict__ P) {
+// 37 KB of LDS and at most 128 VGPRs, so four groups reside per CU.
+__global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restrict__ groups,
+                                                         const uint32_t* __restrict__ W,
+                                                         const float* __restrict__ bmin,
+                                                         const float* __restrict__ bmax, uint32_t N,
+                                                         uint32_t n, int L, int level0,
+                                                         uint32_t* __restrict__ P) {
   __shared__ uint32_t E[kSubT];
-  __shared__ uint32_t E2[kSubT];
   __shared__ uint16_t G[kSubT];
   __shared__ unsigned long long K[kSubT];
   __shared__ uint32_t nodeOff[kSubT / 2 + 1];
   __shared__ uint32_t nodeArg[kSubT / 2];
-  uint32_t* MK = reinterpret_cast<uint32_t*>(K);  // max keys reuse the sort buffer
+  __shared__ uint32_t gmax;  // largest sort group of the level
+  // once the sorted order is read out, the sort buffer holds the sorted ids
+  // (E2) and their max keys (MK)
+  uint32_t* E2 = reinterpret_cast<uint32_t*>(K);
+  uint32_t* MK = E2 + kSubT;
 
   const SubGroup g = groups[blockIdx.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -377,33 +392,102 @@ __global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict
     const int a = level % 3;
     const float* kmin = bmin + (size_t)a * N;
     const float* kmax = bmax + (size_t)a * N;
+    // every thread's key loads are issued together, before the offsets
+    constexpr int kPer = kSubT / kThreads;  // elements per thread
+    uint32_t kv[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      kv[r] = i < m ? fkey(kmin[E[i]]) : 0u;
+    }
     for (uint32_t j = tid; j <= c; j += kThreads) nodeOff[j] = 2u * (uint32_t)desc_count(f, f + j, n);
+    if (tid == 0) gmax = 0;
+    __syncthreads();
+    // Sort groups are the remainders of one node of the previous level: the
+    // ranges of this level's node pairs (2q, 2q+1), contiguous and in order,
+    // with G = q.  When every group fits a power-of-two slot of S entries and
+    // the slots fit the sort buffer, each group is sorted in its own slot
+    // (log2(S) bitonic phases instead of log2(M)); otherwise the whole region
+    // is sorted on (group, key, position).
+    const uint32_t ng = (c + 1) / 2;
+    for (uint32_t q = tid; q < ng; q += kThreads)
+      atomicMax(&gmax, nodeOff[min(2 * q + 2, c)] - nodeOff[2 * q]);
     uint32_t M = 1;
     while (M < m) M <<= 1;
-    for (uint32_t i = tid; i < M; i += kThreads) {
-      K[i] = i < m ? (((unsigned long long)G[i] << 43) |
-                      ((unsigned long long)fkey(kmin[E[i]]) << 11) | i)
-                   : ~0ull;
+    __syncthreads();
+    uint32_t S = 2;
+    while (S < gmax) S <<= 1;
+    const bool seg = (uint64_t)ng * S <= kSubT;
+    const uint32_t span = seg ? ng * S : M;  // sorted entries
+    const uint32_t top = seg ? S : M;        // size of one sorted run
+    for (uint32_t q = tid; q < span; q += kThreads) K[q] = ~0ull;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      if (i >= m) break;
+      const unsigned long long key = kv[r];
+      if (seg) {
+        const uint32_t q = G[i], p = i - nodeOff[2 * q];
+        K[q * S + p] = (key << 11) | p;
+      } else {
+        K[i] = ((unsigned long long)G[i] << 43) | (key << 11) | i;
+      }
     }
     __syncthreads();
-    for (uint32_t k = 2; k <= M; k <<= 1) {
+    for (uint32_t k = 2; k <= top; k <<= 1) {
       for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t t = tid; t < M / 2; t += kThreads) {
+        for (uint32_t t = tid; t < span / 2; t += kThreads) {
           const uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));  // j is a power of two
           const uint32_t ixj = i + j;
-          const bool up = (i & k) == 0;
+          const bool up = (i & k & (top - 1)) == 0;  // every run ascending at k == top
           const unsigned long long x = K[i], y = K[ixj];
           if ((x > y) == up) {
             K[i] = y;
             K[ixj] = x;
           }
         }
-        __syncthreads();
+        // With j <= 64, the pairs of thread t lie in entries [2t - 2t%128,
+        // +128): wave w owns entries 128w + 512r .. +127 in this stage and in
+        // the next one if that also has j <= 64, so a wave barrier orders it.
+        const uint32_t jn = j > 1 ? (j >> 1) : k;  // next stage: (k, j/2) or (2k, k)
+        if ((j == 1 && k == top) || j > 64 || jn > 64) {
+          __syncthreads();
+        } else {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
       }
     }
-    for (uint32_t i = tid; i < m; i += kThreads) E2[i] = E[(uint32_t)(K[i] & 0x7FFull)];
+    const bool last = level == L - 1;
+    uint32_t ev[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      uint32_t e = 0;
+      if (i < m) {
+        if (seg) {
+          const uint32_t q = G[i], gs = nodeOff[2 * q];
+          e = E[gs + (uint32_t)(K[q * S + (i - gs)] & 0x7FFull)];
+        } else {
+          e = E[(uint32_t)(K[i] & 0x7FFull)];
+        }
+      }
+      ev[r] = e;
+      kv[r] = (i < m && !last) ? fkey(kmax[e]) : 0u;  // max keys, issued together
+    }
+    __syncthreads();  // every sorted slot is read: E2 and MK overwrite them
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      if (i < m) {
+        E2[i] = ev[r];
+        MK[i] = kv[r];
+      }
+    }
     __syncthreads();
-    if (level == L - 1) {
+    if (last) {
       for (uint32_t j = tid; j < c; j += kThreads) {
         const uint32_t o = nodeOff[j];
         P[2 * (size_t)(f + j)] = E2[o];
@@ -411,8 +495,6 @@ __global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict
       }
       break;
     }
-    for (uint32_t i = tid; i < m; i += kThreads) MK[i] = fkey(kmax[E2[i]]);
-    __syncthreads();
     for (uint32_t j = wave; j < c; j += kThreads / 64) {
       const uint32_t o = nodeOff[j], e = nodeOff[j + 1];
       uint32_t bk = 0, bp = 0xFFFFFFFFu;
@@ -433,8 +515,26 @@ __global__ __launch_bounds__(kThreads) void k_subtree(const SubGroup* __restrict
       if (lane == 0) nodeArg[j] = (bp > o && bp < e) ? bp : o + 1;  // sizes are >= 2
     }
     __syncthreads();
-    for (uint32_t j = tid; j < c; j += kThreads)
-      write_pair(f + j, E2[nodeOff[j]], E2[nodeArg[j]], level, L, bmin, N, P);
+    {
+      // each thread's pairs (c <= kSubT / 2): last-level keys loaded together
+      constexpr int kPairs = kSubT / 2 / kThreads;
+      const float* knew = bmin + (size_t)((L - 1) % 3) * N;
+      uint32_t px[kPairs], py[kPairs];
+      float fx[kPairs], fy[kPairs];
+#pragma unroll
+      for (int r = 0; r < kPairs; ++r) {
+        const uint32_t j = tid + r * kThreads;
+        px[r] = j < c ? E2[nodeOff[j]] : 0u;
+        py[r] = j < c ? E2[nodeArg[j]] : 0u;
+        fx[r] = j < c ? knew[px[r]] : 0.0f;
+        fy[r] = j < c ? knew[py[r]] : 0.0f;
+      }
+#pragma unroll
+      for (int r = 0; r < kPairs; ++r) {
+        const uint32_t j = tid + r * kThreads;
+        if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bmin, N, P);
+      }
+    }
     for (uint32_t i = tid; i < m; i += kThreads) {
       uint32_t lo = 0, hi = c;  // find j with nodeOff[j] <= i < nodeOff[j+1]
       while (hi - lo > 1) {
