@@ -50,6 +50,7 @@ aur_status aur_scene_add(aur_context* ctx, uint32_t count, const float* vertices
   // take the built arrays; the context's own scene is left unbuilt
   sc->nodes.swap(ctx->nodes);
   sc->root_slab = ctx->root_slab;
+  sc->rec_bytes = ctx->rec_bytes;
   sc->normals.swap(ctx->normals);
   sc->shader_ids.swap(ctx->shader_ids);
   sc->shaders.swap(ctx->shaders);
@@ -112,6 +113,7 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
     J.path_start = start;
     J.n_nodes = sc.count / 2;
     J.root = sc.root_slab;
+    J.rec_bytes = sc.rec_bytes;
     J.num_shaders = sc.num_shaders;
     J.num_lights = sc.num_lights;
     J.nslots = sc.nslots;

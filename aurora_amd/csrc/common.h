@@ -142,6 +142,12 @@ constexpr uint32_t kTagPbr = 0x50425200u;
 // src/kernels/lib/intersect.cuh:106-111), and flags.  96 bytes, 16-byte
 // aligned.  One extra record after the last node (index n) holds the root's
 // own slab (axis 0) in cslab[0..1].
+//
+// A tree's records are kNodeBytes apart, or kNodeLineBytes (zero padding
+// after each record) when the array is too large to stay in the Infinity
+// Cache: unpadded, half the records straddle two 128-byte lines, and once
+// node visits go to HBM every extra line is paid for; while the array is
+// cache-resident the compact form keeps more records in L2.
 constexpr uint32_t kNodeSlabsNormal = 1u;  // every cslab value passes normal_coord
 
 // Coordinates x = 0 or 2^-72 <= |x| <= 2^59.  For a slab value s and a ray
@@ -159,6 +165,16 @@ struct alignas(16) NodeRec {
   uint32_t spare;
 };
 static_assert(sizeof(NodeRec) == 96, "node record");
+constexpr uint32_t kNodeBytes = 96;       // records back to back
+constexpr uint32_t kNodeLineBytes = 128;  // one record per 128-byte line
+
+// Record k of a tree whose records are rec_bytes apart.
+__host__ __device__ inline const NodeRec* node_at(const NodeRec* nodes, uint32_t k, uint32_t rec_bytes) {
+  return reinterpret_cast<const NodeRec*>(reinterpret_cast<const char*>(nodes) + (size_t)k * rec_bytes);
+}
+__host__ __device__ inline NodeRec* node_at(NodeRec* nodes, uint32_t k, uint32_t rec_bytes) {
+  return reinterpret_cast<NodeRec*>(reinterpret_cast<char*>(nodes) + (size_t)k * rec_bytes);
+}
 
 // Subtree arithmetic of a left-complete binary tree of n nodes in heap order.
 // Number of nodes in the subtrees of the consecutive same-depth range [a, b).

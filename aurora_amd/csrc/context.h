@@ -103,6 +103,7 @@ struct JobDev {
   uint32_t w, h, npix, spp, max_depth, rr_start, seed, paths;
   uint32_t rx0, ry0, rw;
   float2 root;  // the tree's root slab (axis 0)
+  uint32_t rec_bytes;  // distance between node records
   aur_camera cam;
 };
 
@@ -110,6 +111,7 @@ struct JobDev {
 struct SceneStore {
   uint32_t count = 0, levels = 0, num_shaders = 0, num_lights = 0, nslots = 0;
   float2 root_slab = make_float2(0.0f, 0.0f);
+  uint32_t rec_bytes = kNodeBytes;
   DevBuf<NodeRec> nodes;
   DevBuf<float> normals;
   DevBuf<uint16_t> shader_ids;
@@ -176,6 +178,7 @@ struct aur_context {
   aur::DevBuf<uint16_t> shader_ids;
   aur::DevBuf<uint32_t> perm;
   aur::DevBuf<aur::NodeRec> nodes;
+  uint32_t rec_bytes = aur::kNodeBytes;  // distance between node records
   float root_lo = 0.0f, root_hi = 0.0f;
 
   // materials and lights

@@ -37,6 +37,7 @@ namespace {
 constexpr uint32_t kTile = 2048;  // radix / emit tile (elements)
 constexpr uint32_t kChunk = 32;   // tiles per chunk of the radix offset scan
 constexpr uint32_t kSubT = 2048;  // max elements of an LDS-resident subtree
+constexpr uint64_t kCompactNodeArrayMax = 128ull << 20;  // bytes
 constexpr int kThreads = 256;
 
 struct TileDesc {
@@ -591,7 +592,16 @@ __device__ inline void slab_of(const float* v, int a, float& lo, float& hi) {
              fmaxf(fmaxf(q[3 * a], q[3 * a + 1]), q[3 * a + 2]));
 }
 
-__global__ void k_node_records(const float* __restrict__ v, uint32_t n,
+// Record k at k * rec_bytes; bytes after the 96 of a record are zero.
+__device__ inline void put_record(NodeRec* __restrict__ nodes, uint32_t k, uint32_t rec_bytes,
+                                  const NodeRec& r) {
+  NodeRec* dst = node_at(nodes, k, rec_bytes);
+  *dst = r;
+  float4* pad = reinterpret_cast<float4*>(dst + 1);
+  for (uint32_t q = 0; q < (rec_bytes - kNodeBytes) / 16; ++q) pad[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void k_node_records(const float* __restrict__ v, uint32_t n, uint32_t rec_bytes,
                                NodeRec* __restrict__ nodes) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > n) return;
@@ -603,7 +613,7 @@ __global__ void k_node_records(const float* __restrict__ v, uint32_t n,
     r.cslab[2] = r.cslab[3] = 0.0f;
     r.flags = 0;
     r.spare = 0;
-    nodes[k] = r;
+    put_record(nodes, k, rec_bytes, r);
     return;
   }
 #pragma unroll
@@ -633,7 +643,7 @@ __global__ void k_node_records(const float* __restrict__ v, uint32_t n,
   for (int c = 0; c < 4; ++c) normal = normal && normal_coord(r.cslab[c]);
   r.flags = normal ? kNodeSlabsNormal : 0u;
   r.spare = 0;
-  nodes[k] = r;
+  put_record(nodes, k, rec_bytes, r);
 }
 
 // cudaGenerateTB (utility.cu:136-180).
@@ -857,13 +867,19 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, ctx->normals.reserve((size_t)N * 9));
   AUR_HIP(ctx, ctx->texcoords.reserve((size_t)N * 6));
   AUR_HIP(ctx, ctx->shader_ids.reserve(N));
-  AUR_HIP(ctx, ctx->nodes.reserve((size_t)n + 1));  // + the root's own slab
+  // records one cache line apart once the compact array would fill more than
+  // half of the 256 MB Infinity Cache (node visits then go to HBM)
+  const uint32_t rec_bytes =
+      (uint64_t)(n + 1) * kNodeBytes > kCompactNodeArrayMax ? kNodeLineBytes : kNodeBytes;
+  AUR_HIP(ctx, ctx->nodes.reserve(((size_t)(n + 1) * rec_bytes + sizeof(NodeRec) - 1) /
+                                  sizeof(NodeRec)));  // + the root's own slab
   k_gather_geometry<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N, ctx->in_vertices.p,
                                                    ctx->in_normals.p, ctx->in_texcoords.p,
                                                    ctx->in_shader_ids.p, ctx->vertices.p,
                                                    ctx->normals.p, ctx->texcoords.p,
                                                    ctx->shader_ids.p);
-  k_node_records<<<blocks((uint64_t)n + 1, 256), 256, 0, s>>>(ctx->vertices.p, n, ctx->nodes.p);
+  k_node_records<<<blocks((uint64_t)n + 1, 256), 256, 0, s>>>(ctx->vertices.p, n, rec_bytes,
+                                                               ctx->nodes.p);
   AUR_LAUNCH_CHECK(ctx, "gather/node records");
   AUR_HIP(ctx, hipEventRecord(e1, s));
   AUR_HIP(ctx, hipEventSynchronize(e1));
@@ -874,9 +890,11 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   // the root's own slab (cslab[0..1] of record n) for launches that start
   // every ray at the root without reading it
   float root[2] = {0.0f, 0.0f};
-  AUR_HIP(ctx, hipMemcpyAsync(root, &ctx->nodes.p[n].cslab[0], sizeof(root), hipMemcpyDeviceToHost, s));
+  AUR_HIP(ctx, hipMemcpyAsync(root, &node_at(ctx->nodes.p, n, rec_bytes)->cslab[0], sizeof(root),
+                              hipMemcpyDeviceToHost, s));
   AUR_HIP(ctx, hipStreamSynchronize(s));
   ctx->root_slab = make_float2(root[0], root[1]);
+  ctx->rec_bytes = rec_bytes;
   ctx->levels = (uint32_t)L;
   ctx->built = true;
   ctx->has_tb = false;

@@ -33,6 +33,7 @@ constexpr int kBlock = 128;
 struct SceneDev {
   const NodeRec* nodes;
   uint32_t n_nodes;
+  uint32_t rec_bytes;
   const float* normals;
   const uint16_t* shader_ids;
   const aur_shader* shaders;
@@ -120,7 +121,7 @@ struct CRay {
 __device__ inline bool occluded(const SceneDev& sc, V3 o, V3 d, float tmax, float2* stk,
                                 uint32_t stride, uint32_t& nrays) {
   ++nrays;
-  return trace_ray<true>(sc.nodes, sc.n_nodes, o, d, tmax, stk, stride).id >= 0;
+  return trace_ray<true>(sc.nodes, sc.n_nodes, sc.rec_bytes, o, d, tmax, stk, stride).id >= 0;
 }
 
 // region: (left, bottom, region width, region pixels); thread l renders
@@ -151,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
     while (depth <= kMaxDepth && ns > 0) {
       CRay ray = stack[--ns];
       ++nrays;
-      const TraceResult hr = trace_ray<false>(sc.nodes, sc.n_nodes, ray.pos, ray.dir, ray.t, stk, stride);
+      const TraceResult hr =
+          trace_ray<false>(sc.nodes, sc.n_nodes, sc.rec_bytes, ray.pos, ray.dir, ray.t, stk, stride);
       if (hr.id < 0) continue;
       ray.t = hr.t;
       hpId = hr.id;
@@ -494,7 +496,8 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
     atomicAdd(&a.stats[1], (unsigned long long)n_sh);
   }
   persistent_trace(
-      TreeRef{a.job.nodes, a.job.n_nodes, a.job.root}, (uint64_t)n_ext + n_sh, a.refill, stack,
+      TreeRef{a.job.nodes, a.job.n_nodes, a.job.root, a.job.rec_bytes}, (uint64_t)n_ext + n_sh,
+      a.refill, stack,
       [&](uint64_t i, V3& o, V3& d, float& tmax, TreeRef& tree, bool& any) {
         float4 o4, d4;
         uint32_t jb = 0;
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uin
         }
         if constexpr (kMulti) {
           const JobDev& J = a.jobs[jb];
-          tree = TreeRef{J.nodes, J.n_nodes, J.root};
+          tree = TreeRef{J.nodes, J.n_nodes, J.root, J.rec_bytes};
         }
         o = mk(o4.x, o4.y, o4.z);
         d = mk(d4.x, d4.y, d4.z);
@@ -1209,6 +1212,7 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   J.path_start = 0;
   J.n_nodes = ctx->count / 2;
   J.root = ctx->root_slab;
+  J.rec_bytes = ctx->rec_bytes;
   J.num_shaders = ctx->num_shaders;
   J.num_lights = ctx->num_lights;
   J.nslots = ctx->num_slots;
@@ -1296,9 +1300,10 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + kStatsOffset);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   if (o->mode == AUR_MODE_COMPAT) {
-    SceneDev sc{ctx->nodes.p, ctx->count / 2, ctx->normals.p, ctx->shader_ids.p, ctx->shaders.p,
-                ctx->num_shaders, ctx->lights.p, ctx->num_lights, ctx->texcoords.p, ctx->textures.p,
-                ctx->texels.p, ctx->num_textures};
+    SceneDev sc{ctx->nodes.p,        ctx->count / 2,    ctx->rec_bytes,   ctx->normals.p,
+                ctx->shader_ids.p,   ctx->shaders.p,    ctx->num_shaders, ctx->lights.p,
+                ctx->num_lights,     ctx->texcoords.p,  ctx->textures.p,  ctx->texels.p,
+                ctx->num_textures};
     const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
     const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
     PassThrottle throttle(s);

@@ -44,7 +44,7 @@ __device__ __forceinline__ void trace_rays(TreeRef tree, const aur_ray* __restri
   uint64_t next = wave * chunk;
   const uint64_t end = next + chunk < n ? next + chunk : n;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const float2 root = root_slab(tree.nodes, tree.n_nodes);  // once per wave
+  const float2 root = root_slab(tree.nodes, tree.n_nodes, tree.rec_bytes);  // once per wave
   Trav s;
   bool active = false;
   uint64_t item = 0;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void trace_rays(TreeRef tree, const aur_ray* __restri
         break;
       }
     }
-    if (active && trav_step<kAny, false>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
+    if (active && trav_step<kAny, false>(s, tree.nodes, tree.n_nodes, tree.rec_bytes, stk, nullptr)) {
       store_result(item, s, hits, ids);
       active = false;
     }
@@ -78,7 +78,7 @@ __device__ __forceinline__ void trace_rays(TreeRef tree, const aur_ray* __restri
 
 template <bool kAny, bool kSplit>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restrict__ nodes,
-                                                        uint32_t n_nodes,
+                                                        uint32_t n_nodes, uint32_t rec_bytes,
                                                         const aur_ray* __restrict__ rays, uint64_t n,
                                                         aur_hit* __restrict__ hits,
                                                         int32_t* __restrict__ ids, uint32_t refill,
@@ -95,16 +95,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_pt(const NodeRec* __restric
                          (glb_vf2*)(gstack + gtid),
                          __builtin_amdgcn_make_buffer_rsrc(gstack, (short)0, (int)bytes, 0x00020000),
                          gtid * 8u, lanes, klds};
-    trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill, stk, hits, ids);
+    trace_rays<kAny>(TreeRef{nodes, n_nodes, {}, rec_bytes}, rays, n, refill, stk, hits, ids);
   } else {
-    trace_rays<kAny>(TreeRef{nodes, n_nodes}, rays, n, refill,
+    trace_rays<kAny>(TreeRef{nodes, n_nodes, {}, rec_bytes}, rays, n, refill,
                      lds_stack(stk_mem + threadIdx.x, blockDim.x), hits, ids);
   }
 }
 
 template <bool kAny>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restrict__ nodes,
-                                                          uint32_t n_nodes,
+                                                          uint32_t n_nodes, uint32_t rec_bytes,
                                                           const aur_ray* __restrict__ rays,
                                                           uint64_t n, aur_hit* __restrict__ hits,
                                                           int32_t* __restrict__ ids) {
@@ -114,16 +114,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_flat(const NodeRec* __restr
   const float4* r = reinterpret_cast<const float4*>(rays + i);
   const float4 a = r[0], b = r[1];
   Trav s;
-  if (trav_init<false>(s, nodes, n_nodes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, nullptr)) {
+  if (trav_init<false>(s, nodes, n_nodes, rec_bytes, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                       nullptr)) {
     const LdsStack stk = lds_stack(stk_mem + threadIdx.x, blockDim.x);
-    while (!trav_step<kAny, false>(s, nodes, n_nodes, stk, nullptr)) {
+    while (!trav_step<kAny, false>(s, nodes, n_nodes, rec_bytes, stk, nullptr)) {
     }
   }
   store_result(i, s, hits, ids);
 }
 
 __global__ __launch_bounds__(kMaxBlock) void k_trace_count(const NodeRec* __restrict__ nodes,
-                                                           uint32_t n_nodes,
+                                                           uint32_t n_nodes, uint32_t rec_bytes,
                                                            const aur_ray* __restrict__ rays,
                                                            uint64_t n, aur_hit* __restrict__ hits,
                                                            unsigned long long* __restrict__ sums) {
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_count(const NodeRec* __rest
   if (i < n) {
     const float4* r = reinterpret_cast<const float4*>(rays + i);
     const float4 a = r[0], b = r[1];
-    const TraceResult res = trace_ray<false, true>(nodes, n_nodes, mk(a.x, a.y, a.z),
+    const TraceResult res = trace_ray<false, true>(nodes, n_nodes, rec_bytes, mk(a.x, a.y, a.z),
                                                    mk(b.x, b.y, b.z), a.w, stk_mem + threadIdx.x,
                                                    blockDim.x, &c);
     reinterpret_cast<float4*>(hits)[i] = make_float4(__int_as_float(res.id), res.t, res.u, res.v);
@@ -207,8 +208,8 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   if (!tp.persistent) {
     const uint64_t nb = (n + block - 1) / block;
     if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
-    k_trace_flat<kAny><<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays,
-                                                                  n, hits, ids);
+    k_trace_flat<kAny><<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2,
+                                                                  ctx->rec_bytes, rays, n, hits, ids);
     AUR_LAUNCH_CHECK(ctx, "k_trace_flat");
     return AUR_OK;
   }
@@ -223,8 +224,8 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
   int per_cu = 0;
   if (tp.refill < 1 || tp.refill > 64)
     return ctx->fail(AUR_ERR_INVALID_ARG, "trace: refill must be in 1..64");
-  using Kern = void (*)(const NodeRec*, uint32_t, const aur_ray*, uint64_t, aur_hit*, int32_t*,
-                        uint32_t, float2*, uint32_t, uint32_t);
+  using Kern = void (*)(const NodeRec*, uint32_t, uint32_t, const aur_ray*, uint64_t, aur_hit*,
+                        int32_t*, uint32_t, float2*, uint32_t, uint32_t);
   const Kern fn = klds != 0 ? k_trace_pt<kAny, true> : k_trace_pt<kAny, false>;
   AUR_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, (int)block, lds_pt));
   if (per_cu < 1) per_cu = 1;
@@ -243,7 +244,7 @@ aur_status launch_trace(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_h
     gstack = ctx->gstack.p;
   }
   hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds_pt, ctx->stream, ctx->nodes.p, ctx->count / 2,
-                     rays, n, hits, ids, tp.refill, gstack, klds, maxrows - rows);
+                     ctx->rec_bytes, rays, n, hits, ids, tp.refill, gstack, klds, maxrows - rows);
   AUR_LAUNCH_CHECK(ctx, "k_trace_pt");
   return AUR_OK;
 }
@@ -262,8 +263,8 @@ aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays
     const uint64_t nb = (n + block - 1) / block;
     if (nb > 0x7FFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "trace: too many rays");
     const size_t lds = (size_t)ctx->levels * block * sizeof(float2);
-    k_trace_count<<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2, rays, n,
-                                                              hits, sums.p);
+    k_trace_count<<<(uint32_t)nb, block, lds, ctx->stream>>>(ctx->nodes.p, ctx->count / 2,
+                                                              ctx->rec_bytes, rays, n, hits, sums.p);
     AUR_LAUNCH_CHECK(ctx, "k_trace_count");
   }
   unsigned long long h[2] = {0, 0};

@@ -196,8 +196,9 @@ __device__ __forceinline__ uint32_t fast_axis(float o, float d) {
 }
 
 // The root's own slab: cslab[0..1] of the record after the last node.
-__device__ __forceinline__ float2 root_slab(const NodeRec* __restrict__ nodes, uint32_t n_nodes) {
-  const float4 rs = reinterpret_cast<const float4*>(nodes + n_nodes)[4];
+__device__ __forceinline__ float2 root_slab(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
+                                            uint32_t rec_bytes) {
+  const float4 rs = reinterpret_cast<const float4*>(node_at(nodes, n_nodes, rec_bytes))[4];
   return make_float2(rs.z, rs.w);
 }
 
@@ -226,9 +227,9 @@ __device__ __forceinline__ bool trav_init_root(Trav& s, float2 root, V3 o, V3 d,
 
 template <bool kCount>
 __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ nodes,
-                                          uint32_t n_nodes, V3 o, V3 d, float tmax,
-                                          TraceCount* cnt) {
-  return trav_init_root<kCount>(s, root_slab(nodes, n_nodes), o, d, tmax, cnt);
+                                          uint32_t n_nodes, uint32_t rec_bytes, V3 o, V3 d,
+                                          float tmax, TraceCount* cnt) {
+  return trav_init_root<kCount>(s, root_slab(nodes, n_nodes, rec_bytes), o, d, tmax, cnt);
 }
 
 // Visits node s.k; returns true when the traversal has finished.  All paths
@@ -239,7 +240,8 @@ __device__ __forceinline__ bool trav_init(Trav& s, const NodeRec* __restrict__ n
 // kMixed: the query kind is the lane's s.any instead of kAny.
 template <bool kAny, bool kCount, bool kMixed = false, class Stk>
 __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ nodes,
-                                          uint32_t n_nodes, const Stk& stk, TraceCount* cnt) {
+                                          uint32_t n_nodes, uint32_t rec_bytes, const Stk& stk,
+                                          TraceCount* cnt) {
   const bool any = kAny || (kMixed && s.any);
   const uint32_t k = s.k;
   if (kCount) {
@@ -249,7 +251,7 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   // the whole record in one round of loads (the right child's slab too, used
   // only when the node has two children: a load issued inside that branch
   // would wait a second memory latency per visit)
-  const float4* rec = reinterpret_cast<const float4*>(nodes + k);
+  const float4* rec = reinterpret_cast<const float4*>(node_at(nodes, k, rec_bytes));
   const float4 a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4];
   const float4 a5 = rec[5];  // right child's slab, flags
   // both triangles up to their determinants, then both reciprocals
@@ -336,12 +338,14 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
   return !found;
 }
 
-// The tree a lane traverses: node records, their count and the root's own
-// slab (cslab[0..1] of the record after the last node).
+// The tree a lane traverses: node records, their count, the root's own
+// slab (cslab[0..1] of the record after the last node) and the distance
+// between records.
 struct TreeRef {
   const NodeRec* nodes;
   uint32_t n_nodes;
   float2 root;
+  uint32_t rec_bytes;
 };
 
 // Persistent-wave traversal of a work range [0, n) of closest-hit and
@@ -393,7 +397,7 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint
         break;
       }
     }
-    if (active && trav_step<false, false, true>(s, tree.nodes, tree.n_nodes, stk, nullptr)) {
+    if (active && trav_step<false, false, true>(s, tree.nodes, tree.n_nodes, tree.rec_bytes, stk, nullptr)) {
       store(item, s);
       active = false;
     }
@@ -404,12 +408,13 @@ __device__ __forceinline__ void persistent_trace(TreeRef tree0, uint64_t n, uint
 // stk: this lane's stack column, stk[depth * stride].
 template <bool kAny, bool kCount = false>
 __device__ __forceinline__ TraceResult trace_ray(const NodeRec* __restrict__ nodes, uint32_t n_nodes,
-                                                 V3 o, V3 d, float tmax, float2* stk,
-                                                 uint32_t stride, TraceCount* cnt = nullptr) {
+                                                 uint32_t rec_bytes, V3 o, V3 d, float tmax,
+                                                 float2* stk, uint32_t stride,
+                                                 TraceCount* cnt = nullptr) {
   Trav s;
   const LdsStack st = lds_stack(stk, stride);
-  if (trav_init<kCount>(s, nodes, n_nodes, o, d, tmax, cnt)) {
-    while (!trav_step<kAny, kCount>(s, nodes, n_nodes, st, cnt)) {
+  if (trav_init<kCount>(s, nodes, n_nodes, rec_bytes, o, d, tmax, cnt)) {
+    while (!trav_step<kAny, kCount>(s, nodes, n_nodes, rec_bytes, st, cnt)) {
     }
   }
   return TraceResult{s.id, s.t, s.u, s.v};
