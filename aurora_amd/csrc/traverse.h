@@ -53,8 +53,25 @@ __device__ __forceinline__ float quot_fast(float n, float d, float y) {
   return __builtin_fmaf(__builtin_fmaf(-d, q, n), y, q);
 }
 
+// v_min_f32 / v_max_f32 on operands that are never signaling NaNs (every
+// operand here is the result of arithmetic, or a copy of one): with IEEE mode
+// on, these instructions return the other operand for a quiet NaN, exactly
+// fminf / fmaxf.  The compiler cannot prove that through the join of the fast
+// and exact quotient paths or the loop-carried range, and would quiet each
+// operand first (one more instruction per operand).
+__device__ __forceinline__ float min_q(float a, float b) {
+  float r;
+  __asm__("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float max_q(float a, float b) {
+  float r;
+  __asm__("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 __device__ __forceinline__ float2 clip(float2 r, float d0, float d1) {
-  return make_float2(fmaxf(fminf(d0, d1), r.x), fminf(fmaxf(d0, d1), r.y));
+  return make_float2(max_q(min_q(d0, d1), r.x), min_q(max_q(d0, d1), r.y));
 }
 
 __device__ __forceinline__ float2 narrow(float2 r, float slo, float shi, float p, float d, float y,
@@ -297,9 +314,10 @@ __device__ __forceinline__ bool trav_step(Trav& s, const NodeRec* __restrict__ n
     const bool okr = (cl + 1 < n_nodes) & (rr.x <= rr.y);
     const bool sw = dc < 0.0f;
     const float2 near_r = sw ? rr : rl, far_r = sw ? rl : rr;
-    const bool nok = sw ? okr : okl, fok = sw ? okl : okr;
-    const bool take_near = nok && near_r.x <= s.t;
-    const bool take_far = !take_near && fok && far_r.x <= s.t;
+    // lane-mask logic rather than selects of booleans
+    const bool nok = (sw & okr) | (!sw & okl), fok = (sw & okl) | (!sw & okr);
+    const bool take_near = nok & (near_r.x <= s.t);
+    const bool take_far = !take_near & fok & (far_r.x <= s.t);
     if (take_near && fok) {
       stk.store((uint32_t)__popc(s.trail), far_r);
       s.trail |= 1u << cd;
