@@ -129,11 +129,65 @@ __host__ __device__ inline U4 philox(U4 c, uint32_t k0, uint32_t k1) {
 }
 __host__ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
-// RNG stream keys: compat mode draws sequentially per pixel sample; PBR mode
-// addresses (pixel, sample, bounce, slot).
+// PBR sample points: Owen-scrambled Sobol points (Burley, "Practical
+// Hash-based Owen Scrambling", JCGT 9(4), 2020).  Each (pixel, dimension
+// pair) has its own scrambles, and the sample index is shuffled by a nested
+// uniform scramble so that dimension pairs are decorrelated; the first 2^m
+// points of each pixel and pair still form a (0,m,2)-net.  Every point is
+// uniformly distributed, so a render is an unbiased estimate, and renders
+// with different seeds are independent randomisations.
+//
+// lowbias32 integer hash (C. Wellons, hash-prospector).
+__host__ __device__ inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__host__ __device__ inline uint32_t rev32(uint32_t x) { return __builtin_bitreverse32(x); }
+// Laine-Karras style permutation with the constants of Burley's paper: each
+// step flips bits from lower bits only, so rev32 around it scrambles a
+// binary fraction from its most significant digit down (Owen scrambling).
+__host__ __device__ inline uint32_t nested_uniform_scramble(uint32_t x, uint32_t seed) {
+  x = rev32(x);
+  x += seed;
+  x ^= x * 0x6c50b47cu;
+  x ^= x * 0xb82f1e52u;
+  x ^= x * 0xc7afe638u;
+  x ^= x * 0x8d22f6e6u;
+  return rev32(x);
+}
+// Sobol dimension 1 (primitive polynomial x + 1, direction numbers m_k =
+// 1, 3, 5, 15, 17, ...): the generator matrix is Pascal's triangle mod 2, so
+// by Lucas' theorem output digit m (from the most significant) is the XOR of
+// index bits n with n a bit superset of m: a superset-sum over GF(2), five
+// masked shift-XOR steps, then a bit reversal.
+__host__ __device__ inline uint32_t sobol_dim1(uint32_t i) {
+  i ^= (i >> 1) & 0x55555555u;
+  i ^= (i >> 2) & 0x33333333u;
+  i ^= (i >> 4) & 0x0F0F0F0Fu;
+  i ^= (i >> 8) & 0x00FF00FFu;
+  i ^= (i >> 16) & 0x0000FFFFu;
+  return rev32(i);
+}
+// Point `sample` of dimension pair `dims` in pixel `pixel`: x, y = 2D
+// Owen-scrambled Sobol; z = a 1D Owen-scrambled van der Corput point with
+// its own shuffle.  Fixed-point in [0, 2^32) (u01 takes the top 24 bits).
+__host__ __device__ inline U4 sobol_owen(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dims) {
+  const uint32_t key = hash32(seed ^ hash32(pixel ^ hash32(dims + 0x9E3779B9u)));
+  const uint32_t i = nested_uniform_scramble(sample, hash32(key ^ 0xA511E9B3u));
+  const uint32_t x = nested_uniform_scramble(rev32(i), hash32(key ^ 0x63D83595u));
+  const uint32_t y = nested_uniform_scramble(sobol_dim1(i), hash32(key ^ 0x8CB92BA7u));
+  const uint32_t j = nested_uniform_scramble(sample, hash32(key ^ 0x2F7A1C3Du));
+  const uint32_t z = nested_uniform_scramble(rev32(j), hash32(key ^ 0x5A3E9F17u));
+  return U4{x, y, z, 0u};
+}
+
+// Compat mode's RNG stream key: it draws sequentially per pixel sample.
 constexpr uint32_t kKeyCompat = 0x2545F491u;
-constexpr uint32_t kKeyPbr = 0x6A09E667u;
-constexpr uint32_t kTagPbr = 0x50425200u;
+
 
 // Traversal node record: the two triangles of node k as (v1, e1 = v2-v1,
 // e2 = v3-v1) -- the same float values Ray::intersect computes on the fly

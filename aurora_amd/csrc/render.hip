@@ -266,9 +266,12 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
 
 // ------------------------------------------------------------------- PBR --
 
+// Sample point of (pixel, sample) for dimension pair (bounce, slot): slot 0
+// of bounce 0 = pixel position, slot 1 = BSDF direction (x, y) and Russian
+// roulette (z), slots 16.. = light samples.
 __device__ inline U4 pbr_draw(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce,
                               uint32_t slot) {
-  return philox(U4{pixel, sample, (bounce << 16) | slot, kTagPbr}, seed, kKeyPbr);
+  return sobol_owen(seed, pixel, sample, (bounce << 16) | slot);
 }
 
 // sin/cos of (pi/4) q, |q| <= 1, Taylor polynomials in Horner form.
@@ -433,18 +436,9 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
   const uint32_t l = r % J.npix, smp = r / J.npix;
   const uint32_t x = J.rx0 + l % J.rw, y = J.ry0 + l / J.rw;
   const uint32_t p = y * J.w + x;  // frame pixel: the random-number key
+  // the pixel's sample points are stratified for every power-of-two prefix
   const U4 rn = pbr_draw(J.seed, p, smp, 0, 0);
-  const float j0 = u01(rn.x), j1 = u01(rn.y);
-  uint32_t sw = 1;
-  while ((sw + 1) * (sw + 1) <= J.spp) ++sw;
-  float fx, fy;
-  if (sw * sw == J.spp) {
-    fx = ((float)(smp % sw) + j0) / (float)sw;
-    fy = ((float)(smp / sw) + j1) / (float)sw;
-  } else {
-    fx = j0;
-    fy = j1;
-  }
+  const float fx = u01(rn.x), fy = u01(rn.y);
   const aur_camera& cam = J.cam;
   const float px = ((float)x + fx) / (float)J.w - 0.5f;
   const float py = ((float)y + fy) / (float)J.h - 0.5f;

@@ -228,7 +228,6 @@ inline const Shader& shader_of(const Scene& sc, uint32_t tri) {
 
 // ---- random numbers ----
 constexpr uint32_t kKeyCompat = 0x2545F491u;
-constexpr uint32_t kKeyPbr = 0x6A09E667u;
 
 struct CompatRng {
   uint32_t seed, pixel, sample, k = 0;
@@ -242,8 +241,54 @@ struct CompatRng {
   }
 };
 
+// PBR sample points: Owen-scrambled Sobol points, Burley, "Practical
+// Hash-based Owen Scrambling", JCGT 9(4), 2020 (nested uniform scrambling by
+// a Laine-Karras permutation between bit reversals, with the paper's
+// constants; the sample index shuffled the same way).  Sobol dimension 0 is
+// the bit-reversed index; dimension 1 (polynomial x + 1, Joe-Kuo direction
+// numbers 1, 3, 5, 15, ...) is written here as the direction-number XOR sum,
+// one index bit at a time.  Keys: lowbias32 (C. Wellons, hash-prospector).
+inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+inline uint32_t rev32(uint32_t x) {
+  uint32_t r = 0;
+  for (int k = 0; k < 32; ++k) r |= ((x >> k) & 1u) << (31 - k);
+  return r;
+}
+inline uint32_t nested_uniform_scramble(uint32_t x, uint32_t seed) {
+  x = rev32(x);
+  x += seed;
+  x ^= x * 0x6c50b47cu;
+  x ^= x * 0xb82f1e52u;
+  x ^= x * 0xc7afe638u;
+  x ^= x * 0x8d22f6e6u;
+  return rev32(x);
+}
+inline uint32_t sobol_dim1(uint32_t i) {
+  uint32_t v = 1u << 31, r = 0;  // v_k = m_k / 2^k as a 32-bit fraction
+  for (int k = 0; k < 32; ++k, v ^= v >> 1)
+    if ((i >> k) & 1u) r ^= v;
+  return r;
+}
+inline u4 sobol_owen(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dims) {
+  const uint32_t key = hash32(seed ^ hash32(pixel ^ hash32(dims + 0x9E3779B9u)));
+  const uint32_t i = nested_uniform_scramble(sample, hash32(key ^ 0xA511E9B3u));
+  const uint32_t x = nested_uniform_scramble(rev32(i), hash32(key ^ 0x63D83595u));
+  const uint32_t y = nested_uniform_scramble(sobol_dim1(i), hash32(key ^ 0x8CB92BA7u));
+  const uint32_t j = nested_uniform_scramble(sample, hash32(key ^ 0x2F7A1C3Du));
+  const uint32_t z = nested_uniform_scramble(rev32(j), hash32(key ^ 0x5A3E9F17u));
+  return u4{x, y, z, 0u};
+}
+// dimension pair (bounce, slot): slot 0 of bounce 0 = pixel position, slot 1
+// = BSDF direction (x, y) and Russian roulette (z), slots 16.. = light samples
 inline u4 pbr_draw(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t slot) {
-  return philox4x32_10(u4{pixel, sample, (bounce << 16) | slot, 0x50425200u}, seed, kKeyPbr);
+  return sobol_owen(seed, pixel, sample, (bounce << 16) | slot);
 }
 
 // ---- camera rays ----
@@ -551,20 +596,10 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
   return Sample{L, covered};
 }
 
-f3 camera_dir_pbr(const Camera& c, uint32_t w, uint32_t h, uint32_t x, uint32_t y, uint32_t spp,
-                  uint32_t seed, uint32_t pixel, uint32_t s) {
+f3 camera_dir_pbr(const Camera& c, uint32_t w, uint32_t h, uint32_t x, uint32_t y, uint32_t seed,
+                  uint32_t pixel, uint32_t s) {
   const u4 r = pbr_draw(seed, pixel, s, 0, 0);
-  const float j0 = u01(r.x), j1 = u01(r.y);
-  uint32_t sw = 1;
-  while ((sw + 1) * (sw + 1) <= spp) ++sw;
-  float fx, fy;
-  if (sw * sw == spp) {
-    fx = ((float)(s % sw) + j0) / (float)sw;
-    fy = ((float)(s / sw) + j1) / (float)sw;
-  } else {
-    fx = j0;
-    fy = j1;
-  }
+  const float fx = u01(r.x), fy = u01(r.y);
   const float px = ((float)x + fx) / (float)w - 0.5f;
   const float py = ((float)y + fy) / (float)h - 0.5f;
   return normalize(px * v3(c.right) * c.tanfov[0] + py * v3(c.up) * c.tanfov[1] + v3(c.forward));
@@ -659,7 +694,7 @@ int orc_render_tex(uint32_t count, const float* verts, const float* normals, con
         CompatRng rng{o.seed, pix, s};
         smp = shade_compat(sc, v3(cam.position), d, rng);
       } else {
-        const f3 d = camera_dir_pbr(cam, w, h, x, y, o.spp, o.seed, pix, s);
+        const f3 d = camera_dir_pbr(cam, w, h, x, y, o.seed, pix, s);
         smp = shade_pbr(sc, o, v3(cam.position), d, pix, s);
       }
       if (smp.covered) {
@@ -711,7 +746,7 @@ int orc_adjoint(uint32_t count, const float* verts, const float* normals, const 
     std::vector<Vertex> path;
     for (uint32_t s = 0; s < o.spp; ++s) {
       path.clear();
-      const f3 d = camera_dir_pbr(cam, w, h, x, y, o.spp, o.seed, (uint32_t)pix, s);
+      const f3 d = camera_dir_pbr(cam, w, h, x, y, o.seed, (uint32_t)pix, s);
       shade_pbr(sc, o, v3(cam.position), d, (uint32_t)pix, s, &path);
       for (int c = 0; c < 3; ++c) {
         const double dl = (double)dimg[pix * 4 + c] / (double)o.spp;
@@ -789,6 +824,15 @@ void orc_tex_sample(const void* desc, const uint8_t* texels, float u, float v, f
   out[2] = c.z;
 }
 void orc_concentric_disk(float u1, float u2, float* out) { concentric_disk(u1, u2, out[0], out[1]); }
+// PBR sample point (x, y, z) of (seed, pixel, sample, dimension pair).
+void orc_sobol_owen(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dims, uint32_t* out) {
+  const u4 r = sobol_owen(seed, pixel, sample, dims);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+uint32_t orc_sobol_dim1(uint32_t i) { return sobol_dim1(i); }
+uint32_t orc_nested_uniform_scramble(uint32_t x, uint32_t seed) { return nested_uniform_scramble(x, seed); }
 void orc_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
   const u4 r = philox4x32_10(u4{ctr[0], ctr[1], ctr[2], ctr[3]}, k0, k1);
   out[0] = r.x;

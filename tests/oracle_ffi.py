@@ -56,6 +56,11 @@ def lib():
         L.orc_tex_sample.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_philox.argtypes = [P, u32, u32, P]
+        L.orc_sobol_owen.argtypes = [u32, u32, u32, u32, P]
+        L.orc_sobol_dim1.argtypes = [u32]
+        L.orc_sobol_dim1.restype = u32
+        L.orc_nested_uniform_scramble.argtypes = [u32, u32]
+        L.orc_nested_uniform_scramble.restype = u32
         _lib = L
     return _lib
 
@@ -250,3 +255,18 @@ def philox(ctr, k0: int, k1: int) -> np.ndarray:
     out = np.zeros(4, dtype=np.uint32)
     lib().orc_philox(_p(c), k0, k1, _p(out))
     return out
+
+
+def sobol_owen(seed: int, pixel: int, sample: int, dims: int) -> np.ndarray:
+    """PBR sample point (x, y, z) as 32-bit fixed-point fractions."""
+    out = np.zeros(3, dtype=np.uint32)
+    lib().orc_sobol_owen(seed, pixel, sample, dims, _p(out))
+    return out
+
+
+def sobol_dim1(i: int) -> int:
+    return int(lib().orc_sobol_dim1(i))
+
+
+def nested_uniform_scramble(x: int, seed: int) -> int:
+    return int(lib().orc_nested_uniform_scramble(x, seed))
