@@ -97,8 +97,9 @@ class MonteCarloRaytracer(Renderer):
     """src/render/monteCarloRaytracer.cpp:23-127 over aur_frame_*.
 
     mode selects the reference-compatible estimator (native.MODE_COMPAT) or
-    the path loop (native.MODE_PBR); seed keys the counter-based RNG that
-    replaces the tick-count-seeded cuRAND states (monteCarloRaytracer.cpp:54).
+    the path loop (native.MODE_PBR); seed keys the stateless random numbers
+    (compat: Philox counters; PBR: Owen-scrambled Sobol points) that replace
+    the tick-count-seeded cuRAND states (monteCarloRaytracer.cpp:54).
     """
 
     def __init__(self, ctx: native.Context, mode: int = native.MODE_COMPAT, max_depth: int = 4,

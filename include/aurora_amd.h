@@ -94,7 +94,7 @@ typedef struct aur_render_opts {
   uint32_t spp;         /* samples per pixel (the reference's Dim::depth) */
   uint32_t max_depth;   /* PBR: bounces per path */
   uint32_t rr_start;    /* PBR: first bounce that applies Russian roulette */
-  uint32_t seed;        /* counter-based RNG key */
+  uint32_t seed;        /* compat: Philox key; PBR: scrambling seed of the Sobol sample points */
 } aur_render_opts;
 
 typedef struct aur_build_stats {
@@ -245,8 +245,9 @@ typedef struct aur_render_params {
 aur_status aur_set_render_params(aur_context* ctx, const aur_render_params* params);
 aur_status aur_get_render_params(aur_context* ctx, aur_render_params* params);
 /* Renders samples [sample_begin, sample_end) of opts->spp into the existing
- * accumulation (progressive rendering and resume; the counter RNG makes a
- * split render equal to an unsplit one). */
+ * accumulation (progressive rendering and resume; every random number is a
+ * function of (seed, pixel, sample, draw), so a split render equals an
+ * unsplit one). */
 aur_status aur_frame_render_samples(aur_context* ctx, const aur_camera* camera,
                                     const aur_render_opts* opts, uint32_t sample_begin,
                                     uint32_t sample_end);
