@@ -22,8 +22,10 @@
 // PBR mode is new semantics that sits beside compat mode: a path loop of
 // max_depth bounces with next-event estimation on every light, cosine
 // sampling of the Lambert BSDF (concentric map, Shirley & Chiu 1997, as in
-// sampling.cuh:51-82), Russian roulette from bounce rr_start, and the same
-// counter-based RNG.  The same float operations run on the GPU.
+// sampling.cuh:51-82), Russian roulette from bounce rr_start, and
+// Owen-scrambled Sobol sample points (Burley, JCGT 2020; sobol_owen below)
+// instead of independent draws.  The same float and integer operations run
+// on the GPU.
 //
 // cudaGenerateTB (utility.cu:136-180) is restated as orc_generate_tb.
 #include <cmath>
