@@ -57,17 +57,18 @@ struct SubGroup {
   uint32_t off, size, first, count;
 };
 
+// Triangle bounds on one axis, (min, max), stored [axis][triangle].
 // kx, ky: the min keys of x and y on the axis of the last level (L - 1),
 // loaded by the caller; level <= L - 2.
 __device__ inline void write_pair_k(uint32_t h, uint32_t x, uint32_t y, float kx, float ky,
-                                    int level, int L, const float* __restrict__ bmin, uint32_t N,
+                                    int level, int L, const float2* __restrict__ bb, uint32_t N,
                                     uint32_t* __restrict__ P) {
   bool y_first = ky < kx;
   if (!y_first && !(kx < ky)) {
     const int lo = max(level + 1, L - 3);
     for (int lev = L - 2; lev >= lo; --lev) {
-      const float* km = bmin + (size_t)(lev % 3) * N;
-      const float ax = km[x], ay = km[y];
+      const float2* km = bb + (size_t)(lev % 3) * N;
+      const float ax = km[x].x, ay = km[y].x;
       if (ay < ax) {
         y_first = true;
         break;
@@ -80,32 +81,33 @@ __device__ inline void write_pair_k(uint32_t h, uint32_t x, uint32_t y, float kx
 }
 
 __device__ inline void write_pair(uint32_t h, uint32_t x, uint32_t y, int level, int L,
-                                  const float* __restrict__ bmin, uint32_t N,
+                                  const float2* __restrict__ bb, uint32_t N,
                                   uint32_t* __restrict__ P) {
-  const float* km = bmin + (size_t)((L - 1) % 3) * N;
-  write_pair_k(h, x, y, km[x], km[y], level, L, bmin, N, P);
+  const float2* km = bb + (size_t)((L - 1) % 3) * N;
+  write_pair_k(h, x, y, km[x].x, km[y].x, level, L, bb, N, P);
 }
 
-// computeTriangleBounds (rebuilder.cu:72-91), SoA [axis][triangle].
-__global__ void k_bounds(const float* __restrict__ v, uint32_t N, float* __restrict__ bmin,
-                         float* __restrict__ bmax) {
+// computeTriangleBounds (rebuilder.cu:72-91), (min, max) pairs [axis][triangle]:
+// one 8-byte load gives both keys of a triangle on an axis.
+__global__ void k_bounds(const float* __restrict__ v, uint32_t N, float2* __restrict__ bb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const float* p = v + (size_t)i * 9;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    bmin[(size_t)a * N + i] = fminf(fminf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]);
-    bmax[(size_t)a * N + i] = fmaxf(fmaxf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]);
-  }
+  for (int a = 0; a < 3; ++a)
+    bb[(size_t)a * N + i] = make_float2(fminf(fminf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]),
+                                        fmaxf(fmaxf(p[3 * a], p[3 * a + 1]), p[3 * a + 2]));
 }
 
-// Identity order, and (when K is given) the first level's axis-0 sort keys.
-__global__ void k_iota(uint32_t* __restrict__ W, uint32_t N, const float* __restrict__ key0,
+// Identity order with the axis-0 max keys (the sort's payload: triangle id,
+// max key), and (when K is given) the first level's axis-0 sort keys.
+__global__ void k_iota(uint2* __restrict__ V, uint32_t N, const float2* __restrict__ bb0,
                        uint32_t* __restrict__ K) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  W[i] = i;
-  if (K) K[i] = fkey(key0[i]);
+  const float2 b = bb0[i];
+  V[i] = make_uint2(i, fkey(b.y));
+  if (K) K[i] = fkey(b.x);
 }
 
 // Indices read back from device arrays are clamped to [0, N) before they
@@ -189,15 +191,19 @@ __global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const ChunkDes
 // Stable scatter, staged through LDS.  Elements of a tile are ranked in order
 // within their digit: rounds of 256 elements; inside a wave, lanes with the
 // same digit are found with 8 ballots; per-wave digit counts go through LDS in
-// wave order.  Each element lands in LDS at its digit-sorted tile position
-// (the tile's own exclusive digit offsets, from its histogram), and the tile
-// is then written out in that order: a digit's run is one contiguous store.
+// wave order.  Each element (key and 8-byte payload) lands in LDS at its
+// digit-sorted tile position (the tile's own exclusive digit offsets, from
+// its histogram), and the tile is then written out in that order: a digit's
+// run is one contiguous store.  (Staging only a 16-bit tile index and
+// re-reading the payload at write-out measured 6% slower despite the
+// smaller LDS footprint.)
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
-    const uint32_t* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ H,
+    const uint2* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ H,
     const uint32_t* __restrict__ O, uint32_t m, uint32_t* __restrict__ Kout,
-    uint32_t* __restrict__ Vout) {
-  __shared__ uint32_t sk[kTile], sv[kTile];
+    uint2* __restrict__ Vout) {
+  __shared__ uint32_t sk[kTile];
+  __shared__ uint2 sv[kTile];
   __shared__ uint32_t base[256];   // next free tile position per digit
   __shared__ uint32_t delta[256];  // output position - tile position, per digit
   __shared__ uint32_t wcnt[kThreads / 64][256];
@@ -225,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t i = r0 + tid;
     const bool valid = i < t.count;
     const uint32_t k = valid ? Kin[t.start + i] : 0u;
-    const uint32_t v = valid ? Vin[t.start + i] : 0u;
+    const uint2 v = valid ? Vin[t.start + i] : make_uint2(0u, 0u);
     const uint32_t dg = (k >> shift) & 255u;
     uint64_t mask = __ballot(valid);
 #pragma unroll
@@ -267,9 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 // restricted to this tile: (key, position), key 0 = none.
 __global__ __launch_bounds__(kThreads) void k_argmax_partial(const TileDesc* __restrict__ tiles,
                                                              const ChildDesc* __restrict__ children,
-                                                             const uint32_t* __restrict__ W,
-                                                             uint32_t N,
-                                                             const float* __restrict__ kmax,
+                                                             const uint2* __restrict__ V,
                                                              uint2* __restrict__ partial) {
   __shared__ uint32_t sk[kThreads / 64], sp[kThreads / 64];
   const TileDesc t = tiles[blockIdx.x];
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void k_argmax_partial(const TileDesc* __r
   const uint32_t lo = max(t.start, c.off + 1), hi = t.start + t.count;
   uint32_t bk = 0, bp = 0xFFFFFFFFu;
   for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) {
-    const uint32_t k = fkey(kmax[clampi(W[i], N)]);
+    const uint32_t k = V[i].y;  // the element's max key, carried by the sort
     if (k > bk) {
       bk = k;
       bp = i;
@@ -334,29 +338,35 @@ __global__ __launch_bounds__(kThreads) void k_argmax_combine(const ChildDesc* __
 }
 
 // Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders;
-// with Knext, also the next level's sort keys of the compacted remainders.
+// with Knext, also the next level's sort keys and max keys of the compacted
+// remainders (one 8-byte load of their bounds on the next axis).
 __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ tiles,
                                                    const ChildDesc* __restrict__ children,
                                                    const uint32_t* __restrict__ argpos,
-                                                   const uint32_t* __restrict__ W,
-                                                   uint32_t* __restrict__ Wnext, uint32_t first,
-                                                   int level, int L, const float* __restrict__ bmin,
+                                                   const uint2* __restrict__ V,
+                                                   uint2* __restrict__ Vnext, uint32_t first,
+                                                   int level, int L, const float2* __restrict__ bb,
                                                    uint32_t N, uint32_t* __restrict__ P,
                                                    uint32_t* __restrict__ Knext) {
   const TileDesc t = tiles[blockIdx.x];
   const ChildDesc c = children[t.child];
-  const float* knext = bmin + (size_t)((level + 1) % 3) * N;
+  const float2* bnext = bb + (size_t)((level + 1) % 3) * N;
   uint32_t am = argpos[t.child];
   if (am <= c.off || am >= c.off + c.size) am = c.off + 1;  // cannot happen for sizes >= 2
   for (uint32_t r = threadIdx.x; r < t.count; r += kThreads) {
     const uint32_t i = t.start + r;
     if (i == c.off) {
-      write_pair(first + t.child, W[c.off], W[am], level, L, bmin, N, P);
+      write_pair(first + t.child, clampi(V[c.off].x, N), clampi(V[am].x, N), level, L, bb, N, P);
     } else if (i >= c.off + 2) {
-      const uint32_t val = (i == am) ? W[c.off + 1] : W[i];
+      const uint32_t val = clampi((i == am) ? V[c.off + 1].x : V[i].x, N);
       const uint32_t dst = i - 2 * (t.child + 1);
-      Wnext[dst] = val;
-      if (Knext) Knext[dst] = fkey(knext[clampi(val, N)]);
+      if (Knext) {
+        const float2 b = bnext[val];
+        Knext[dst] = fkey(b.x);
+        Vnext[dst] = make_uint2(val, fkey(b.y));
+      } else {
+        Vnext[dst] = make_uint2(val, 0u);
+      }
     }
   }
 }
@@ -364,9 +374,8 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
 // One workgroup per group: all remaining levels of the group's subtree(s).
 // 37 KB of LDS and at most 128 VGPRs, so four groups reside per CU.
 __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restrict__ groups,
-                                                         const uint32_t* __restrict__ W,
-                                                         const float* __restrict__ bmin,
-                                                         const float* __restrict__ bmax, uint32_t N,
+                                                         const uint2* __restrict__ V,
+                                                         const float2* __restrict__ bb, uint32_t N,
                                                          uint32_t n, int L, int level0,
                                                          uint32_t* __restrict__ P) {
   __shared__ uint32_t E[kSubT];
@@ -384,22 +393,21 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   uint32_t m = g.size;
   for (uint32_t i = tid; i < m; i += kThreads) {
-    E[i] = clampi(W[g.off + i], N);
+    E[i] = clampi(V[g.off + i].x, N);
     G[i] = 0;
   }
   uint32_t f = g.first, c = g.count;
   __syncthreads();
   for (int level = level0; level < L && m > 0; ++level) {
     const int a = level % 3;
-    const float* kmin = bmin + (size_t)a * N;
-    const float* kmax = bmax + (size_t)a * N;
+    const float2* ba = bb + (size_t)a * N;  // (min, max) on this level's axis
     // every thread's key loads are issued together, before the offsets
     constexpr int kPer = kSubT / kThreads;  // elements per thread
     uint32_t kv[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
       const uint32_t i = tid + r * kThreads;
-      kv[r] = i < m ? fkey(kmin[E[i]]) : 0u;
+      kv[r] = i < m ? fkey(ba[E[i]].x) : 0u;
     }
     for (uint32_t j = tid; j <= c; j += kThreads) nodeOff[j] = 2u * (uint32_t)desc_count(f, f + j, n);
     if (tid == 0) gmax = 0;
@@ -476,7 +484,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
         }
       }
       ev[r] = e;
-      kv[r] = (i < m && !last) ? fkey(kmax[e]) : 0u;  // max keys, issued together
+      kv[r] = (i < m && !last) ? fkey(ba[e].y) : 0u;  // max keys, issued together
     }
     __syncthreads();  // every sorted slot is read: E2 and MK overwrite them
 #pragma unroll
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
     {
       // each thread's pairs (c <= kSubT / 2): last-level keys loaded together
       constexpr int kPairs = kSubT / 2 / kThreads;
-      const float* knew = bmin + (size_t)((L - 1) % 3) * N;
+      const float2* knew = bb + (size_t)((L - 1) % 3) * N;
       uint32_t px[kPairs], py[kPairs];
       float fx[kPairs], fy[kPairs];
 #pragma unroll
@@ -527,13 +535,13 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
         const uint32_t j = tid + r * kThreads;
         px[r] = j < c ? E2[nodeOff[j]] : 0u;
         py[r] = j < c ? E2[nodeArg[j]] : 0u;
-        fx[r] = j < c ? knew[px[r]] : 0.0f;
-        fy[r] = j < c ? knew[py[r]] : 0.0f;
+        fx[r] = j < c ? knew[px[r]].x : 0.0f;
+        fy[r] = j < c ? knew[py[r]].x : 0.0f;
       }
 #pragma unroll
       for (int r = 0; r < kPairs; ++r) {
         const uint32_t j = tid + r * kThreads;
-        if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bmin, N, P);
+        if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bb, N, P);
       }
     }
     for (uint32_t i = tid; i < m; i += kThreads) {
@@ -783,18 +791,18 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   }
 
   // ---- device buffers ----
-  DevBuf<float> bmin, bmax;
-  DevBuf<uint32_t> Wa, Wb, Ka, Kb, H, O, CS, CB, argpos;
+  DevBuf<float2> bb;          // (min, max) per axis and triangle
+  DevBuf<uint2> Va, Vb;        // sort payload: (triangle id, max key on the level's axis)
+  DevBuf<uint32_t> Ka, Kb, H, O, CS, CB, argpos;
   DevBuf<uint2> partial;
   DevBuf<TileDesc> dtiles;
   DevBuf<ChildDesc> dchildren;
   DevBuf<GroupDesc> dgroups;
   DevBuf<ChunkDesc> dchunks;
   DevBuf<SubGroup> dsub;
-  AUR_HIP(ctx, bmin.reserve((size_t)3 * N));
-  AUR_HIP(ctx, bmax.reserve((size_t)3 * N));
-  AUR_HIP(ctx, Wa.reserve(N));
-  AUR_HIP(ctx, Wb.reserve(N));
+  AUR_HIP(ctx, bb.reserve((size_t)3 * N));
+  AUR_HIP(ctx, Va.reserve(N));
+  AUR_HIP(ctx, Vb.reserve(N));
   AUR_HIP(ctx, ctx->perm.reserve(N));
   if (!plan.empty()) {
     AUR_HIP(ctx, Ka.reserve(N));
@@ -824,21 +832,21 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, hipEventCreate(&e1));
   AUR_HIP(ctx, hipEventRecord(e0, s));
 
-  k_bounds<<<blocks(N, 256), 256, 0, s>>>(ctx->in_vertices.p, N, bmin.p, bmax.p);
+  k_bounds<<<blocks(N, 256), 256, 0, s>>>(ctx->in_vertices.p, N, bb.p);
   // level 0's keys come with the identity order; every later level's keys
-  // are gathered by the previous level's emit pass
-  k_iota<<<blocks(N, 256), 256, 0, s>>>(Wa.p, N, bmin.p, plan.empty() ? nullptr : Ka.p);
+  // and max keys are gathered by the previous level's emit pass
+  k_iota<<<blocks(N, 256), 256, 0, s>>>(Va.p, N, bb.p, plan.empty() ? nullptr : Ka.p);
   k_fill0<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N);
-  uint32_t* Wcur = Wa.p;
-  uint32_t* Walt = Wb.p;
+  uint2* Vcur = Va.p;
+  uint2* Valt = Vb.p;
   for (size_t li = 0; li < plan.size(); ++li) {
     const LevelPlan& lp = plan[li];
-    const int a = lp.level % 3;
     const TileDesc* tl = dtiles.p + lp.tile_base;
     const ChildDesc* ch = dchildren.p + lp.child_base;
     const GroupDesc* gr = dgroups.p + lp.group_base;
     const ChunkDesc* ck = dchunks.p + lp.chunk_base;
-    uint32_t *kin = Ka.p, *kout = Kb.p, *vin = Wcur, *vout = Walt;
+    uint32_t *kin = Ka.p, *kout = Kb.p;
+    uint2 *vin = Vcur, *vout = Valt;
     for (int pass = 0; pass < 4; ++pass) {
       const uint32_t shift = 8u * pass;
       k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, shift, H.p);
@@ -849,17 +857,17 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
       std::swap(kin, kout);
       std::swap(vin, vout);
     }
-    // after 4 passes the sorted data is back in (Ka, Wcur); Ka is free again
-    k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Wcur, N, bmax.p + (size_t)a * N, partial.p);
+    // after 4 passes the sorted data is back in (Ka, Vcur); Ka is free again
+    k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Vcur, partial.p);
     k_argmax_combine<<<blocks(lp.nchildren, kThreads / 64), kThreads, 0, s>>>(ch, lp.nchildren,
                                                                               partial.p, argpos.p);
-    k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Wcur, Walt, lp.first, lp.level, L,
-                                          bmin.p, N, ctx->perm.p,
+    k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Vcur, Valt, lp.first, lp.level, L,
+                                          bb.p, N, ctx->perm.p,
                                           li + 1 < plan.size() ? Ka.p : nullptr);
-    std::swap(Wcur, Walt);
+    std::swap(Vcur, Valt);
   }
   AUR_LAUNCH_CHECK(ctx, "nmh level kernels");
-  k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Wcur, bmin.p, bmax.p, N, n, L, level0,
+  k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Vcur, bb.p, N, n, L, level0,
                                                       ctx->perm.p);
   AUR_LAUNCH_CHECK(ctx, "k_subtree");
 
