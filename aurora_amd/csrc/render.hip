@@ -454,17 +454,15 @@ __global__ void k_pbr_camera(PbrArgs a, uint32_t* __restrict__ queue) {
 }
 
 // One persistent launch for bounce k: closest hit for every path entering
-// bounce k (rays in queue order from ray_o/ray_d + roff, paths in queue qext,
-// items [0, n_ext); hits in queue order) and any hit for every shadow ray
-// shading appended at bounce k - 1 (the shadow queue, paths of queue qsh,
-// items [n_ext, n_ext + n_sh)); both counts in word w = k % 3.  parts bit 0:
+// bounce k (rays in queue order from ray_o/ray_d + roff, items [0, n_ext);
+// hits in queue order) and any hit for every shadow ray shading appended at
+// bounce k - 1 (the dense shadow queue, items [n_ext, n_ext + n_sh)); both
+// counts in word w = k % 3.  parts bit 0:
 // extension rays, bit 1: shadow rays.  The two sets are independent, so one
 // launch overlaps their tails.
 template <bool kMulti, bool kSplit>
 __global__ __launch_bounds__(kBlock) void k_pbr_trace(PbrArgs a, uint32_t w, uint32_t parts,
-                                                      uint32_t roff,
-                                                      const uint32_t* __restrict__ qext,
-                                                      const uint32_t* __restrict__ qsh) {
+                                                      uint32_t roff) {
   extern __shared__ float2 stk_mem[];
   const auto stack = [&]() {
     if constexpr (kSplit) {
@@ -1011,13 +1009,13 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
   // bounce b: shade, then one launch tracing bounce b + 1's extension rays
   // with bounce b's shadow rays, then bounce b's next-event resolve
   const uint32_t sh = a.nslots ? 2u : 0u;
-  const auto trace = [&](uint32_t w, uint32_t parts, uint32_t roff, const uint32_t* qe, const uint32_t* qs) {
+  const auto trace = [&](uint32_t w, uint32_t parts, uint32_t roff) {
     if (a.gstack)
-      k_pbr_trace<kMulti, true><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff, qe, qs);
+      k_pbr_trace<kMulti, true><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff);
     else
-      k_pbr_trace<kMulti, false><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff, qe, qs);
+      k_pbr_trace<kMulti, false><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff);
   };
-  trace(0u, 1u, 0u, fr.queue[0].p, fr.queue[1].p);
+  trace(0u, 1u, 0u);
   for (uint32_t b = 0; b < a.max_depth; ++b) {
     const uint32_t w = b % 3u, wn = (b + 1u) % 3u;
     uint32_t* qin = fr.queue[b & 1u].p;
@@ -1028,7 +1026,7 @@ void pbr_pass(aur_context* ctx, PbrArgs& a, size_t lds, uint32_t grid_trace,
       k_pbr_shade<kMulti, false><<<nblocks(cap, 128), 128, 0, s>>>(a, w, b, qin, qout, a.jobs);
     const uint32_t parts = (b + 1u < a.max_depth ? 1u : 0u) | sh;
     if (parts)
-      trace(wn, parts, ((b + 1u) & 1u) * cap, qout, qin);
+      trace(wn, parts, ((b + 1u) & 1u) * cap);
     if (a.nslots) {
       if (adj)
         k_pbr_resolve<true><<<nblocks(cap, 256), 256, 0, s>>>(a, w, b, qin);
