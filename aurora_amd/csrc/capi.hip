@@ -232,10 +232,8 @@ aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* 
     AUR_HIP(ctx, hipMemcpy(ctx->lights.p, lights, count * sizeof(aur_light), hipMemcpyHostToDevice));
   }
   ctx->num_lights = count;
-  // next-event slots per path vertex: area lights use `samples` slots, others one
   ctx->num_slots = 0;
-  for (uint32_t i = 0; i < count; ++i)
-    ctx->num_slots += lights[i].type == AUR_LIGHT_AREA ? (uint32_t)std::max<int>(1, lights[i].samples) : 1u;
+  for (uint32_t i = 0; i < count; ++i) ctx->num_slots += aur::light_slots(lights[i]);
   return AUR_OK;
 }
 

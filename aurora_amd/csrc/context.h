@@ -15,6 +15,14 @@
 
 namespace aur {
 
+// Next-event slots a light takes per path vertex in PBR mode: an area light
+// one per sample, an ambient light two (its unshadowed ambient part and its
+// shadowed positional part), point and directional lights one.
+__host__ __device__ inline uint32_t light_slots(const aur_light& L) {
+  if (L.type == AUR_LIGHT_AREA) return L.samples > 1 ? (uint32_t)L.samples : 1u;
+  return L.type == AUR_LIGHT_AMBIENT ? 2u : 1u;
+}
+
 // The device allocator in force (aur_set_alloc_hook); alloc == nullptr means
 // hipMalloc / hipFree.
 aur_alloc_hook current_alloc_hook();

@@ -621,9 +621,27 @@ __device__ __forceinline__ ShadeOut shade_path(const PbrArgs& a, const JobDev& J
       a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
     } else {
+      // ambient: the fraction `ambient` lights unshadowed; the rest comes from
+      // the light's position with no falloff and a shadow ray (compat mode's
+      // estimateDirectRadianceDelta, radiance.cuh:13-28)
       const V3 c = bf * (CI * Lt.ambient);
       a.sh_c[base + e * ss] = make_float4(c.x, c.y, c.z, Lt.ambient * kInvPi);
       a.sh_d[base + e * ss] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);  // no shadow ray
+      ++e;
+      V3 wi = ld3(Lt.position) - P;
+      const float dist = length(wi);
+      wi = wi / dist;
+      const float cosS = dot(N, wi);
+      const float rest = 1.0f - Lt.ambient;
+      float flag = 0.0f;
+      if (cosS > 0.0f && rest != 0.0f) {
+        const V3 c2 = bf * ((CI * rest) * cosS);
+        a.sh_o[base + e * ss] = make_float4(P.x, P.y, P.z, dist);
+        a.sh_c[base + e * ss] = make_float4(c2.x, c2.y, c2.z, rest * cosS * kInvPi);
+        flag = 1.0f;
+        out.shadows |= 1ull << e;
+      }
+      a.sh_d[base + e * ss] = make_float4(wi.x, wi.y, wi.z, flag);
       ++e;
     }
   }
@@ -871,7 +889,7 @@ __global__ __launch_bounds__(256) void k_adj_sweep(PbrArgs a, const float4* __re
           uint32_t s = 0;
           for (uint32_t li = 0; li < J.num_lights; ++li) {
             const aur_light& Lt = J.lights[li];
-            const uint32_t ns = Lt.type == AUR_LIGHT_AREA ? (uint32_t)max(1, (int)Lt.samples) : 1u;
+            const uint32_t ns = light_slots(Lt);
             float W = 0.0f;
             for (uint32_t k = 0; k < ns; ++k, ++s) W += a.rec_w[((size_t)b * a.nslots + s) * stride + q];
             if (W != 0.0f) {

@@ -314,6 +314,43 @@ def area_light(corner, e1, e2, direction, radiance, samples=1) -> np.void:
     return L
 
 
+def _light(kind, color, intensity) -> np.void:
+    # fields every light type sets (src/core/scene.cpp:340-345)
+    L = np.zeros((), dtype=LIGHT_DTYPE)
+    L["type"] = kind
+    L["color"] = color
+    L["intensity"] = intensity
+    L["samples"] = 1
+    return L
+
+
+def point_light(position, color, intensity=1.0) -> np.void:
+    """Point light record (src/core/scene.cpp:364-368): radiant intensity
+    color * intensity, falling off as 1/d^2."""
+    L = _light(LIGHT_POINT, color, intensity)
+    L["position"] = position
+    return L
+
+
+def directional_light(direction, color, intensity=1.0) -> np.void:
+    """Directional light record (src/core/scene.cpp:370-374): `direction` is
+    the direction the light travels (unit length)."""
+    L = _light(LIGHT_DIRECTIONAL, color, intensity)
+    d = np.asarray(direction, dtype=np.float64)
+    L["direction"] = d / np.linalg.norm(d)
+    return L
+
+
+def ambient_light(position, color, ambient, intensity=1.0) -> np.void:
+    """Ambient light record (src/core/scene.cpp:356-362): the fraction
+    `ambient` (the ambient shade) lights every point unshadowed, the rest
+    comes from `position` with a shadow ray."""
+    L = _light(LIGHT_AMBIENT, color, intensity)
+    L["position"] = position
+    L["ambient"] = ambient
+    return L
+
+
 def make_camera(position, direction, tanfov) -> np.ndarray:
     """Camera of src/util/camera.h:18-27 in float32 (tanfov given directly)."""
     f32 = np.float32

@@ -24,8 +24,10 @@
 // sampling of the Lambert BSDF (concentric map, Shirley & Chiu 1997, as in
 // sampling.cuh:51-82), Russian roulette from bounce rr_start, and
 // Owen-scrambled Sobol sample points (Burley, JCGT 2020; sobol_owen below)
-// instead of independent draws.  The same float and integer operations run
-// on the GPU.
+// instead of independent draws.  Point, directional and ambient lights keep
+// compat mode's formulas (direct_delta), an ambient light with both its
+// unshadowed and its positional part.  The same float and integer operations
+// run on the GPU.
 //
 // cudaGenerateTB (utility.cu:136-180) is restated as orc_generate_tb.
 #include <cmath>
@@ -574,8 +576,22 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
           }
         }
       } else {
+        // ambient part unshadowed, the rest from the position without falloff
+        // (direct_delta's ambient case)
         L = L + bf * (CI * Lt.ambient);
         if (rec) rec->back().W[li] += Lt.ambient;
+        f3 wi = v3(Lt.position) - P;
+        const float dist = length(wi);
+        wi = wi / dist;
+        const float cosS = dot(N, wi);
+        const float rest = 1.0f - Lt.ambient;
+        if (cosS > 0.0f && rest != 0.0f) {
+          Ray sr{P, wi, dist, 1.0f, 1.0f};
+          if (!occluded(sc, sr)) {
+            L = L + bf * ((CI * rest) * cosS);
+            if (rec) rec->back().W[li] += rest * cosS;
+          }
+        }
       }
     }
     if (b + 1 >= o.max_depth) break;

@@ -14,6 +14,12 @@
   irradiance (which is itself checked against FP64 quadrature).
 * PBR bounce throughput and Russian roulette: furnace (closed emissive
   sphere), expected radiance emission * sum albedo^k.
+* Point, directional and ambient lights (radiance.cuh:13-28 in compat mode,
+  the same formulas in PBR mode) on an open floor at depth 1, against their
+  closed forms: exact per pixel in compat mode (deterministic sub-pixel
+  positions); in PBR mode within 2e-3 and z-tested against an 8x8 quadrature
+  of the pixel footprint, and exact for a directional light (constant over
+  the floor).
 """
 from __future__ import annotations
 
@@ -243,3 +249,68 @@ def test_furnace_roulette_unbiased(oracle):
     se = v.std() / math.sqrt(v.size)
     assert abs(v.mean() - expected) < 5 * se + 1e-6, (v.mean(), expected, se)
     assert v.std() > 0  # roulette does randomise the estimate
+
+
+def _delta_closed_form(X, L, albedo):
+    """Lambert floor (normal +y) lit by one delta light: albedo/pi times
+    CI/d^2 cos (point), CI cos (directional), CI (a + (1 - a) cos) (ambient,
+    positional part without falloff)."""
+    f = albedo / math.pi
+    CI = float(L["color"][0]) * float(L["intensity"])
+    kind = int(L["type"])
+    if kind == scenes.LIGHT_DIRECTIONAL:
+        return np.full(len(X), f * CI * max(0.0, -float(L["direction"][1])))
+    w = np.asarray(L["position"], np.float64) - X
+    d = np.linalg.norm(w, axis=1)
+    cos = np.maximum(0.0, w[:, 1] / d)
+    if kind == scenes.LIGHT_POINT:
+        return f * CI * cos / d ** 2
+    a = float(L["ambient"])
+    return f * CI * (a + (1.0 - a) * cos)
+
+
+DELTA_LIGHTS = {
+    "point": lambda: scenes.point_light((0.5, 2.0, 0.3), (1.0, 1.0, 1.0), 6.0),
+    "directional": lambda: scenes.directional_light((0.3, -1.0, 0.2), (2.0, 2.0, 2.0)),
+    "ambient": lambda: scenes.ambient_light((-1.0, 1.5, 1.0), (1.5, 1.5, 1.5), 0.3),
+}
+
+
+@pytest.mark.parametrize("kind", list(DELTA_LIGHTS))
+def test_delta_lights_closed_form(oracle, kind):
+    albedo = 0.5
+    L = DELTA_LIGHTS[kind]()
+    s = scenes.plane_under_light(albedo=albedo)
+    s.lights = np.array([L], dtype=scenes.LIGHT_DTYPE)
+    tree = oracle.tree_arrays(s, oracle.nmh_build(s.vertices))
+    w, h = s.width, s.height
+    pix = np.arange(w * h, dtype=np.uint32)
+
+    img = oracle.render(tree, s, 0, 16, 1, 3, 11, pixels=pix)
+    assert np.all(img[:, 3] == 1.0)
+    exp = np.empty(pix.size)
+    for q, p in enumerate(pix):
+        x, y = p % w, p // w
+        i = np.arange(16)
+        sub = np.stack([x / w - 0.5 + (i % 4) * (1 / w) / 4, y / h - 0.5 + (i // 4) * (1 / h) / 4], 1)
+        exp[q] = _delta_closed_form(_floor_hits(s, sub), L, albedo).mean()
+    assert exp.min() > 0.01
+    np.testing.assert_allclose(img[:, 0], exp, rtol=1e-5)
+    np.testing.assert_array_equal(img[:, 0], img[:, 1])
+
+    spp = 64
+    img = oracle.render(tree, s, 1, spp, 1, 3, 11, pixels=pix)
+    g = (np.arange(8) + 0.5) / 8
+    gx, gy = (a.ravel() for a in np.meshgrid(g, g))
+    z = []
+    for q, p in enumerate(pix):
+        x, y = p % w, p // w
+        v = _delta_closed_form(_floor_hits(s, np.stack([(x + gx) / w - 0.5, (y + gy) / h - 0.5], 1)), L, albedo)
+        assert abs(img[q, 0] - v.mean()) <= 2e-3 * v.mean()
+        sd = math.sqrt(v.var() / spp)
+        if kind == "directional":
+            assert img[q, 0] == pytest.approx(v.mean(), rel=1e-5)
+        else:
+            z.append((img[q, 0] - v.mean()) / sd)
+    if z:
+        assert np.abs(z).max() < 5.0
