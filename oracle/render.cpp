@@ -842,6 +842,16 @@ void orc_tex_sample(const void* desc, const uint8_t* texels, float u, float v, f
   out[2] = c.z;
 }
 void orc_concentric_disk(float u1, float u2, float* out) { concentric_disk(u1, u2, out[0], out[1]); }
+// refract (src/util/math.h:235-246) on float[3] vectors; out is written only
+// when it returns 1 (no total internal reflection)
+int orc_refract(const float* i, const float* n, float eta1, float eta2, float* out) {
+  f3 r;
+  if (!refract(r, v3(i), v3(n), eta1, eta2)) return 0;
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  return 1;
+}
 // PBR sample point (x, y, z) of (seed, pixel, sample, dimension pair).
 void orc_sobol_owen(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dims, uint32_t* out) {
   const u4 r = sobol_owen(seed, pixel, sample, dims);

@@ -55,6 +55,8 @@ def lib():
         L.orc_render_tex.restype = i32
         L.orc_tex_sample.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
+        L.orc_refract.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
+        L.orc_refract.restype = ctypes.c_int
         L.orc_philox.argtypes = [P, u32, u32, P]
         L.orc_sobol_owen.argtypes = [u32, u32, u32, u32, P]
         L.orc_sobol_dim1.argtypes = [u32]
@@ -248,6 +250,15 @@ def concentric_disk(u1: float, u2: float):
     out = np.zeros(2, dtype=np.float32)
     lib().orc_concentric_disk(u1, u2, _p(out))
     return out
+
+
+def refract(i, n, eta1: float, eta2: float):
+    """The reference's refract: float32[3] direction, or None on total
+    internal reflection."""
+    iv = np.ascontiguousarray(i, dtype=np.float32)
+    nv = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    return out if lib().orc_refract(_p(iv), _p(nv), eta1, eta2, _p(out)) else None
 
 
 def philox(ctr, k0: int, k1: int) -> np.ndarray:

@@ -314,3 +314,47 @@ def test_delta_lights_closed_form(oracle, kind):
             z.append((img[q, 0] - v.mean()) / sd)
     if z:
         assert np.abs(z).max() < 5.0
+
+
+def test_refract_reference_form(oracle):
+    """src/util/math.h:235-246 returns r = eta i - (eta + cos t) n for the
+    incoming direction i and the normal n on its side.  Pinned from the
+    geometry of Snell's law (float64, angles through trig): total internal
+    reflection exactly past the critical angle, the tangential part eta i_t,
+    the normal part eta (n.i) - eta - cos t, and -(2 eta + 1) n at normal
+    incidence.  Away from normal incidence r is not parallel to Snell's
+    direction; compat images keep that form."""
+    rng = np.random.default_rng(8)
+    for eta1, eta2 in [(1.0, 1.5), (1.5, 1.0), (1.0, 1.33), (2.4, 1.0)]:
+        eta = eta1 / eta2
+        for _ in range(200):
+            n = rng.normal(size=3)
+            n /= np.linalg.norm(n)
+            t = rng.normal(size=3)
+            t -= t.dot(n) * n
+            t /= np.linalg.norm(t)
+            theta = rng.uniform(0.0, 0.5 * math.pi - 1e-3)
+            i = (math.sin(theta) * t - math.cos(theta) * n).astype(np.float32)
+            nf = n.astype(np.float32)
+            i64, n64 = i.astype(np.float64), nf.astype(np.float64)
+            sin_i = np.linalg.norm(np.cross(i64, n64))
+            r = oracle.refract(i, nf, eta1, eta2)
+            if eta * sin_i > 1.0 + 1e-5:
+                assert r is None
+                continue
+            if eta * sin_i > 1.0 - 1e-5:
+                continue  # within float rounding of the critical angle
+            assert r is not None
+            cos_t = math.cos(math.asin(eta * sin_i))
+            ni = n64.dot(i64)
+            r64 = r.astype(np.float64)
+            np.testing.assert_allclose(r64 - r64.dot(n64) * n64, eta * (i64 - ni * n64), atol=2e-6)
+            assert r64.dot(n64) == pytest.approx(eta * ni - eta - cos_t, abs=5e-6)
+            assert r64.dot(n64) < 0.0
+    n = np.array([0.0, 1.0, 0.0], np.float32)
+    np.testing.assert_allclose(oracle.refract(-n, n, 1.0, 1.5), -(2 / 1.5 + 1) * n, rtol=1e-6)
+    i = np.array([math.sqrt(0.5), -math.sqrt(0.5), 0.0], np.float32)
+    r = oracle.refract(i, n, 1.0, 1.5).astype(np.float64)
+    sin_t = math.sqrt(0.5) / 1.5
+    snell = np.array([sin_t, -math.sqrt(1 - sin_t ** 2), 0.0])
+    assert np.linalg.norm(np.cross(r / np.linalg.norm(r), snell)) > 0.1
