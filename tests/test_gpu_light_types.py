@@ -5,7 +5,9 @@ within 1e-4 relative; the adjoint's albedo and light gradients against the
 oracle's within 1e-4 of the largest, and each light's within 1e-3 of its
 own largest component; each delta light's radiance gradient against central
 differences of the GPU forward render, 1%.  Light gradients are with respect
-to radiance (colour * intensity), so the intensities are not 1 here.
+to radiance (colour * intensity), so the intensities are not 1 here.  A batch
+whose scenes take different next-event slot counts gives every job its
+single-frame image bit for bit.
 """
 from __future__ import annotations
 
@@ -118,3 +120,41 @@ def test_delta_light_gradients_match_central_differences(gpu_ctx):
             assert fd == pytest.approx(want, rel=1e-2), l
     finally:
         gpu_ctx.upload_lights(s.lights)
+
+
+def test_batch_with_delta_lights_equals_single_renders(gpu_ctx):
+    """Jobs whose scenes take different next-event slot counts (mixed lights:
+    2 area + point + directional + ambient = 6 slots; ambient alone = 2;
+    a Cornell box with 2 area lights = 2) give their single-frame images."""
+    import torch
+
+    mixed, amb = _scene("mixed"), _scene("ambient")
+    box = scenes.cornell(seed=4, n_spheres=2, subdivisions=1, n_lights=2)
+    specs = [(mixed, W, H, 4, 4, 41), (amb, 20, 16, 3, 3, 42), (box, 16, 16, 2, 5, 43), (mixed, 16, 8, 5, 2, 44)]
+    singles = []
+    for s, w, h, spp, depth, seed in specs:
+        gpu_ctx.upload_scene(s)
+        gpu_ctx.rebuild_nmh()
+        gpu_ctx.frame_create(w, h, spp)
+        gpu_ctx.frame_render(s.camera, native.Context.opts(native.MODE_PBR, spp, depth, 2, seed))
+        singles.append(gpu_ctx.frame_read())
+    ctx = native.Context(0)
+    try:
+        ids = {}
+        for s, *_ in specs:
+            if id(s) not in ids:
+                ids[id(s)] = ctx.scene_add(s)
+        jobs = native.Context.make_jobs(
+            dict(scene=ids[id(s)], width=w, height=h, spp=spp, max_depth=depth, rr_start=2, seed=seed,
+                 camera=s.camera) for s, w, h, spp, depth, seed in specs)
+        total = sum(w * h for _, w, h, *_ in specs)
+        out = torch.full((total, 4), float("nan"), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        ctx.batch_render_ptr(jobs, out.data_ptr())
+        img = out.cpu().numpy()
+    finally:
+        ctx.close()
+    for j, ref in zip(jobs, singles):
+        o, n = int(j["out_offset"]), int(j["width"]) * int(j["height"])
+        assert ref[:, :3].max() > 0
+        assert np.array_equal(img[o:o + n].view(np.uint32), ref.view(np.uint32))
