@@ -1,7 +1,8 @@
 """Shading branches the other replay scenes leave out: translucent shaders
 (compat mode's refraction rays, src/kernels/monteCarloRaytracer.cu's ray
 stack with refract, src/util/math.h:235-246), shaders that are both mirror
-and translucent (two secondary rays from one hit), and area lights with
+and translucent (two secondary rays from one hit), emissive shaders in
+compat mode (the hit's Li starts from emissionColor), and area lights with
 several shadow samples (one next-event slot each in PBR mode).  Per-pixel
 replay against the oracle within 1e-4 relative in both modes, and the
 adjoint against the oracle's within 1e-4 of the largest gradient.
@@ -22,9 +23,10 @@ def glass_scene(translucent=True):
     s = scenes.cornell(seed=6, n_spheres=3, subdivisions=2, width=W, height=H, n_lights=2,
                        split_materials=True)
     sh = s.shaders
+    # the last six shaders are the spheres' halves: glass, mirror-glass, emissive
+    a = sh.shape[0] - 6
+    sh["emission_color"][a + 4:a + 6] = (0.6, 0.5, 0.2)
     if translucent:
-        # the last six shaders are the spheres' halves: glass, mirror-glass, plain
-        a = sh.shape[0] - 6
         sh["translucence"][a:a + 2] = 0.7
         sh["refractive_index"][a:a + 2] = 1.5
         sh["reflectivity"][a + 2:a + 4] = 0.3
@@ -51,9 +53,13 @@ def test_secondary_rays_replay_oracle(gpu_ctx, oracle, mode):
     ref = oracle.render(tree, s, mode, 8, 4, 2, 31)
     den = np.maximum(np.abs(ref), 1e-6)
     assert (np.abs(img - ref) / den).max() <= 1e-4
-    if mode == native.MODE_COMPAT:  # the glass changes what the camera sees
+    if mode == native.MODE_COMPAT:  # the glass and the emission change what the camera sees
         plain = oracle.render(tree, glass_scene(translucent=False), mode, 8, 4, 2, 31)
         assert (np.abs(plain - ref).max(axis=1) > 1e-3).mean() > 0.05
+        dark = glass_scene()
+        dark.shaders["emission_color"] = 0.0
+        unlit = oracle.render(tree, dark, mode, 8, 4, 2, 31)
+        assert (np.abs(unlit - ref).max(axis=1) > 1e-3).mean() > 0.01
 
 
 def test_multi_sample_lights_adjoint(gpu_ctx, oracle):
