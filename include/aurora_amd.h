@@ -55,7 +55,12 @@ typedef struct aur_shader {
 } aur_shader;
 
 /* Light record, field for field the reference's Light (80 B,
- * src/util/light.h:14-51; built at src/core/scene.cpp:328-396). */
+ * src/util/light.h:14-51; built at src/core/scene.cpp:328-396).  Radiance is
+ * color * intensity.  Point: position, falling off as 1/d^2.  Directional:
+ * direction = the way the light travels.  Ambient: the fraction `ambient`
+ * lights every point unshadowed, the rest comes from position with a shadow
+ * ray and no falloff.  Area: corner position, edges e1 and e2, emitting
+ * along direction, area = |e1| |e2|, `samples` shadow rays per hit. */
 enum { AUR_LIGHT_AMBIENT = 0, AUR_LIGHT_POINT = 1, AUR_LIGHT_DIRECTIONAL = 2, AUR_LIGHT_AREA = 3 };
 typedef struct aur_light {
   int32_t type;
