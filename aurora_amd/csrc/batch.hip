@@ -89,6 +89,8 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
     if ((uint64_t)j.width * j.height * j.spp > 0x7FFFFFFFull)
       return ctx->fail(AUR_ERR_INVALID_ARG, "batch: a job's pixels * spp exceeds 2^31");
     if (ctx->scenes[j.scene]->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "batch: scene without shaders");
+    if (ctx->scenes[j.scene]->nslots > aur::kMaxSlots)
+      return ctx->fail(AUR_ERR_UNSUPPORTED, "batch: a scene has more than 64 next-event slots (light samples) per vertex");
   }
   AUR_HIP(ctx, hipSetDevice(ctx->device));
   // shortest jobs first

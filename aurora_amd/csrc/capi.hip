@@ -384,8 +384,10 @@ aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera, const au
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (!camera || !opts) return ctx->fail(AUR_ERR_INVALID_ARG, "render: null argument");
   if (opts->abi_version != AUR_ABI_VERSION) return ctx->fail(AUR_ERR_UNSUPPORTED, "render: ABI version");
+  aur_status st = aur::frame_render_check(ctx, opts, 0, opts->spp);  // a refused render keeps the frame
+  if (st != AUR_OK) return st;
   AUR_HIP(ctx, hipSetDevice(ctx->device));
-  aur_status st = aur_frame_clear(ctx);
+  st = aur_frame_clear(ctx);
   if (st != AUR_OK) return st;
   return aur::frame_render(ctx, camera, opts, 0, opts->spp);
 }

@@ -15,6 +15,9 @@
 
 namespace aur {
 
+// PBR mode carries a vertex's next-event slots in one 64-bit mask.
+constexpr uint32_t kMaxSlots = 64;
+
 // Next-event slots a light takes per path vertex in PBR mode: an area light
 // one per sample, an ambient light two (its unshadowed ambient part and its
 // shadowed positional part), point and directional lights one.
@@ -258,6 +261,8 @@ aur_status division_check(aur_context* ctx, uint64_t samples, uint64_t seed, uin
                           uint64_t* recip_bad, uint64_t* quot_bad);
 aur_status trace_closest_count(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits,
                                uint64_t* popped, uint64_t* passed);
+// Argument and state checks of frame_render; no work is queued.
+aur_status frame_render_check(aur_context* ctx, const aur_render_opts* opts, uint32_t s0, uint32_t s1);
 aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,
                         uint32_t s0, uint32_t s1);
 aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_render_opts* opts,

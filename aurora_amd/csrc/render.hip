@@ -1071,7 +1071,7 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
   FrameState& fr = ctx->frame;
   hipStream_t s = ctx->stream;
   const uint32_t cap = run.cap;
-  if (run.nslots > 64)
+  if (run.nslots > kMaxSlots)
     return ctx->fail(AUR_ERR_UNSUPPORTED, "render: more than 64 next-event slots (light samples) per vertex");
   aur_status st = pbr_reserve(ctx, cap, run.nslots, run.max_depth, adj != nullptr);
   if (st != AUR_OK) return st;
@@ -1293,19 +1293,29 @@ aur_status pbr_run(aur_context* ctx, const PbrRun& run, unsigned long long* stat
                         : pbr_passes<true>(ctx, run, stats, nullptr);
 }
 
-aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o,
-                        uint32_t s0, uint32_t s1) {
-  FrameState& fr = ctx->frame;
+aur_status frame_render_check(aur_context* ctx, const aur_render_opts* o, uint32_t s0, uint32_t s1) {
+  const FrameState& fr = ctx->frame;
   if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "render: geometry not built");
   if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "render: no frame");
   if (ctx->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "render: no shaders uploaded");
   if (o->spp == 0 || s1 > o->spp || s0 > s1) return ctx->fail(AUR_ERR_INVALID_ARG, "render: bad sample range");
   if (o->mode == AUR_MODE_PBR && o->max_depth == 0) return ctx->fail(AUR_ERR_INVALID_ARG, "render: max_depth 0");
   if (o->mode != AUR_MODE_COMPAT && o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "render: mode");
+  if (o->mode == AUR_MODE_PBR && ctx->num_slots > kMaxSlots)
+    return ctx->fail(AUR_ERR_UNSUPPORTED, "render: more than 64 next-event slots (light samples) per vertex");
+  const uint32_t npix = (fr.right - fr.left + 1) * (fr.top - fr.bottom + 1);
+  if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
+  return AUR_OK;
+}
+
+aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o,
+                        uint32_t s0, uint32_t s1) {
+  const aur_status chk = frame_render_check(ctx, o, s0, s1);
+  if (chk != AUR_OK) return chk;
+  FrameState& fr = ctx->frame;
   hipStream_t s = ctx->stream;
   const uint32_t w = fr.width, h = fr.height;
   const uint32_t rw = fr.right - fr.left + 1, rh = fr.top - fr.bottom + 1, npix = rw * rh;
-  if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
   AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + kStatsOffset);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
@@ -1360,6 +1370,8 @@ aur_status frame_adjoint(aur_context* ctx, const aur_camera* cam, const aur_rend
   if (fr.width == 0) return ctx->fail(AUR_ERR_STATE, "adjoint: no frame");
   if (ctx->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "adjoint: no shaders uploaded");
   if (o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: PBR mode only");
+  if (ctx->num_slots > kMaxSlots)
+    return ctx->fail(AUR_ERR_UNSUPPORTED, "adjoint: more than 64 next-event slots (light samples) per vertex");
   if (o->spp == 0 || o->max_depth == 0) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: spp and max_depth must be > 0");
   if (o->max_depth > kAdjMaxDepth) return ctx->fail(AUR_ERR_INVALID_ARG, "adjoint: max_depth > 16");
   if (!d_image || !g_albedo || (ctx->num_lights && !g_light))

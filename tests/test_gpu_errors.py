@@ -185,3 +185,39 @@ def test_degenerate_triangles_match_oracle(gpu_ctx, oracle):
     any_g, _ = gpu_ctx.trace(rays, any_hit=True)
     any_o, _ = oracle.trace(v[perm], rays, any_hit=True)
     assert np.array_equal(any_g, any_o)
+
+
+def test_next_event_slot_limit(fresh):
+    """PBR mode carries at most 64 next-event slots per path vertex, and an
+    ambient light takes two: 32 ambient lights render, 33 are refused as
+    unsupported in PBR mode (the context keeps working) and render in compat
+    mode, which has no such limit.  Batches of such a scene are refused too."""
+    import torch
+
+    s = scenes.cornell(width=16, height=16, subdivisions=1)
+    fresh.upload_scene(s)
+    fresh.rebuild_nmh()
+    fresh.frame_create(16, 16, 2)
+    amb = scenes.ambient_light((2.5, 4.0, 2.5), (0.02, 0.02, 0.02), 0.5)
+    pbr = native.Context.opts(native.MODE_PBR, 2)
+    fresh.upload_lights(np.array([amb] * 32, dtype=scenes.LIGHT_DTYPE))
+    fresh.frame_render(s.camera, pbr)
+    first = fresh.frame_read()
+    assert first[:, :3].max() > 0
+    many = np.array([amb] * 33, dtype=scenes.LIGHT_DTYPE)
+    fresh.upload_lights(many)
+    assert _status(fresh.frame_render, s.camera, pbr) == UNSUPPORTED
+    assert np.array_equal(fresh.frame_read().view(np.uint32), first.view(np.uint32))  # nothing ran
+    fresh.frame_render(s.camera, native.Context.opts(native.MODE_COMPAT, 2))
+    assert fresh.frame_read()[:, :3].max() > 0
+    fresh.upload_lights(np.array([amb] * 32, dtype=scenes.LIGHT_DTYPE))
+    fresh.frame_render(s.camera, pbr)
+    assert np.array_equal(fresh.frame_read().view(np.uint32), first.view(np.uint32))
+    # a kept scene takes the context's built tree, so the batch part comes last
+    s.lights = many
+    sid = fresh.scene_add(s)
+    jobs = native.Context.make_jobs([dict(scene=sid, width=8, height=8, spp=1, max_depth=2, rr_start=2,
+                                          seed=1, camera=s.camera)])
+    out = torch.zeros((64, 4), dtype=torch.float32, device="cuda")
+    assert _status(fresh.batch_render_ptr, jobs, out.data_ptr()) == UNSUPPORTED
+    assert float(out.abs().sum()) == 0.0
