@@ -85,3 +85,17 @@ def test_odd_counts_are_padded():
     assert s.count == 3  # random_soup does not pad; generators built by _finish do
     c = scenes.cornell(n_spheres=1, subdivisions=0)
     assert c.count % 2 == 0
+
+
+def test_delta_light_records():
+    p = scenes.point_light((1.0, 2.0, 3.0), (0.5, 0.25, 1.0), 8.0)
+    assert p["type"] == scenes.LIGHT_POINT and p["intensity"] == 8.0 and p["samples"] == 1
+    assert np.array_equal(p["position"], np.float32([1.0, 2.0, 3.0]))
+    assert p["area"] == 0.0 and p["ambient"] == 0.0  # src/core/scene.cpp:344-345
+    d = scenes.directional_light((0.0, -3.0, 4.0), (1.0, 1.0, 1.0))
+    assert d["type"] == scenes.LIGHT_DIRECTIONAL
+    assert np.allclose(d["direction"], [0.0, -0.6, 0.8]) and abs(np.linalg.norm(d["direction"]) - 1) < 1e-6
+    a = scenes.ambient_light((0.0, 5.0, 0.0), (1.0, 0.9, 0.8), 0.45, 2.0)
+    assert a["type"] == scenes.LIGHT_AMBIENT and a["ambient"] == np.float32(0.45) and a["intensity"] == 2.0
+    recs = np.array([p, d, a], dtype=scenes.LIGHT_DTYPE)
+    assert recs.itemsize == 80 and recs["pad"].max() == 0
