@@ -97,8 +97,9 @@ typedef struct aur_render_opts {
   uint32_t abi_version; /* AUR_ABI_VERSION */
   uint32_t mode;        /* AUR_MODE_COMPAT: the reference's MC ray tracer; AUR_MODE_PBR: path loop */
   uint32_t spp;         /* samples per pixel (the reference's Dim::depth) */
-  uint32_t max_depth;   /* PBR: bounces per path */
-  uint32_t rr_start;    /* PBR: first bounce that applies Russian roulette */
+  uint32_t max_depth;   /* PBR: bounces per path (compat mode ignores it: the reference's
+                           fixed MAX_DEPTH 8 of mirror and refraction rays) */
+  uint32_t rr_start;    /* PBR: first bounce that applies Russian roulette (ignored in compat) */
   uint32_t seed;        /* compat: Philox key; PBR: scrambling seed of the Sobol sample points */
 } aur_render_opts;
 
