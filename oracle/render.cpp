@@ -492,6 +492,17 @@ inline void onb(f3 n, f3& t1, f3& t2) {
   t2 = mk3(b, sign + n.y * n.y * a, -n.y);
 }
 
+// Cosine-weighted direction around N: the concentric disk point lifted to
+// the hemisphere (Malley's method) in the onb() frame.
+inline f3 sample_cosine(f3 N, float u1, float u2) {
+  float dx, dy;
+  concentric_disk(u1, u2, dx, dy);
+  const float dz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
+  f3 t1, t2;
+  onb(N, t1, t2);
+  return t1 * dx + t2 * dy + N * dz;
+}
+
 // One path vertex as the adjoint needs it: shader index, Russian-roulette
 // probability applied after it (1 when none), and per light the unoccluded
 // next-event weight W with contribution beta * albedo * radiance * W / pi.
@@ -602,12 +613,7 @@ Sample shade_pbr(const Scene& sc, const Opts& o, f3 origin, f3 dir, uint32_t pix
       beta = beta / q;
       if (rec) rec->back().q = q;
     }
-    float dx, dy;
-    concentric_disk(u01(r.x), u01(r.y), dx, dy);
-    const float dz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
-    f3 t1, t2;
-    onb(N, t1, t2);
-    const f3 wi = t1 * dx + t2 * dy + N * dz;
+    const f3 wi = sample_cosine(N, u01(r.x), u01(r.y));
     beta = beta * albedo;
     ray = Ray{P, wi, kInfinity, 1.0f, 1.0f};
   }
@@ -842,6 +848,17 @@ void orc_tex_sample(const void* desc, const uint8_t* texels, float u, float v, f
   out[2] = c.z;
 }
 void orc_concentric_disk(float u1, float u2, float* out) { concentric_disk(u1, u2, out[0], out[1]); }
+// onb and sample_cosine of the PBR path loop on float[3] vectors
+void orc_onb(const float* n, float* t1, float* t2) {
+  f3 a, b;
+  onb(v3(n), a, b);
+  t1[0] = a.x, t1[1] = a.y, t1[2] = a.z;
+  t2[0] = b.x, t2[1] = b.y, t2[2] = b.z;
+}
+void orc_sample_cosine(const float* n, float u1, float u2, float* out) {
+  const f3 w = sample_cosine(v3(n), u1, u2);
+  out[0] = w.x, out[1] = w.y, out[2] = w.z;
+}
 // refract (src/util/math.h:235-246) on float[3] vectors; out is written only
 // when it returns 1 (no total internal reflection)
 int orc_refract(const float* i, const float* n, float eta1, float eta2, float* out) {

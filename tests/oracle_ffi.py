@@ -56,6 +56,8 @@ def lib():
         L.orc_tex_sample.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_refract.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
+        L.orc_onb.argtypes = [P, P, P]
+        L.orc_sample_cosine.argtypes = [P, ctypes.c_float, ctypes.c_float, P]
         L.orc_refract.restype = ctypes.c_int
         L.orc_philox.argtypes = [P, u32, u32, P]
         L.orc_sobol_owen.argtypes = [u32, u32, u32, u32, P]
@@ -259,6 +261,20 @@ def refract(i, n, eta1: float, eta2: float):
     nv = np.ascontiguousarray(n, dtype=np.float32)
     out = np.zeros(3, dtype=np.float32)
     return out if lib().orc_refract(_p(iv), _p(nv), eta1, eta2, _p(out)) else None
+
+
+def onb(n):
+    nv = np.ascontiguousarray(n, dtype=np.float32)
+    t1, t2 = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_onb(_p(nv), _p(t1), _p(t2))
+    return t1, t2
+
+
+def sample_cosine(n, u1: float, u2: float) -> np.ndarray:
+    nv = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_sample_cosine(_p(nv), u1, u2, _p(out))
+    return out
 
 
 def philox(ctr, k0: int, k1: int) -> np.ndarray:

@@ -358,3 +358,47 @@ def test_refract_reference_form(oracle):
     sin_t = math.sqrt(0.5) / 1.5
     snell = np.array([sin_t, -math.sqrt(1 - sin_t ** 2), 0.0])
     assert np.linalg.norm(np.cross(r / np.linalg.norm(r), snell)) > 0.1
+
+
+def _normals(rng, k):
+    n = rng.normal(size=(k, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    poles = np.array([[0, 0, 1], [0, 0, -1], [0, 1e-4, -1], [1e-3, 0, -1], [1, 0, 0], [0, 1, 0]], np.float64)
+    poles /= np.linalg.norm(poles, axis=1, keepdims=True)
+    return np.concatenate([n, poles]).astype(np.float32)
+
+
+def test_onb_orthonormal_right_handed(oracle):
+    """Duff et al.'s basis (JCGT 2017) around unit normals, both poles
+    included: t1, t2, n orthonormal with t1 x t2 = n."""
+    for n in _normals(np.random.default_rng(12), 500):
+        t1, t2 = (v.astype(np.float64) for v in oracle.onb(n))
+        n64 = n.astype(np.float64)
+        for v in (t1, t2):
+            assert abs(np.linalg.norm(v) - 1.0) < 1e-5
+        assert max(abs(t1 @ t2), abs(t1 @ n64), abs(t2 @ n64)) < 1e-5
+        np.testing.assert_allclose(np.cross(t1, t2), n64, atol=1e-5)
+
+
+def test_cosine_sampling_distribution(oracle):
+    """Cosine-weighted hemisphere sampling (the PBR path loop's BSDF sample):
+    over one jittered point per cell of a 128x128 grid of (u1, u2) the
+    directions are unit, lie in n's hemisphere, cos^2 theta is uniform on
+    [0, 1] and the azimuth is uniform -- the two marginals that define the
+    cosine density."""
+    rng = np.random.default_rng(13)
+    k = 128
+    cell = np.stack(np.meshgrid(np.arange(k), np.arange(k)), -1).reshape(-1, 2)
+    for n in _normals(rng, 6):
+        u = (cell + rng.uniform(size=cell.shape)) / k
+        t1, t2 = (v.astype(np.float64) for v in oracle.onb(n))
+        n64 = n.astype(np.float64)
+        w = np.array([oracle.sample_cosine(n, a, b) for a, b in u], np.float64)
+        assert np.abs(np.linalg.norm(w, axis=1) - 1.0).max() < 1e-5
+        c = w @ n64
+        assert c.min() > -1e-6
+        h, _ = np.histogram(np.clip(c, 0, 1) ** 2, bins=16, range=(0, 1))
+        assert np.abs(h / len(w) - 1 / 16).max() < 0.004
+        phi = np.arctan2(w @ t2, w @ t1)
+        h, _ = np.histogram(phi, bins=16, range=(-math.pi, math.pi))
+        assert np.abs(h / len(w) - 1 / 16).max() < 0.004
