@@ -859,6 +859,11 @@ void orc_sample_cosine(const float* n, float u1, float u2, float* out) {
   const f3 w = sample_cosine(v3(n), u1, u2);
   out[0] = w.x, out[1] = w.y, out[2] = w.z;
 }
+// reflect (src/util/math.h:231-234) on float[3] vectors
+void orc_reflect(const float* i, const float* n, float* out) {
+  const f3 r = reflect(v3(i), v3(n));
+  out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
 // refract (src/util/math.h:235-246) on float[3] vectors; out is written only
 // when it returns 1 (no total internal reflection)
 int orc_refract(const float* i, const float* n, float eta1, float eta2, float* out) {

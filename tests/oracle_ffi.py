@@ -57,6 +57,7 @@ def lib():
         L.orc_concentric_disk.argtypes = [ctypes.c_float, ctypes.c_float, P]
         L.orc_refract.argtypes = [P, P, ctypes.c_float, ctypes.c_float, P]
         L.orc_onb.argtypes = [P, P, P]
+        L.orc_reflect.argtypes = [P, P, P]
         L.orc_sample_cosine.argtypes = [P, ctypes.c_float, ctypes.c_float, P]
         L.orc_refract.restype = ctypes.c_int
         L.orc_philox.argtypes = [P, u32, u32, P]
@@ -261,6 +262,14 @@ def refract(i, n, eta1: float, eta2: float):
     nv = np.ascontiguousarray(n, dtype=np.float32)
     out = np.zeros(3, dtype=np.float32)
     return out if lib().orc_refract(_p(iv), _p(nv), eta1, eta2, _p(out)) else None
+
+
+def reflect(i, n) -> np.ndarray:
+    iv = np.ascontiguousarray(i, dtype=np.float32)
+    nv = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_reflect(_p(iv), _p(nv), _p(out))
+    return out
 
 
 def onb(n):

@@ -402,3 +402,16 @@ def test_cosine_sampling_distribution(oracle):
         phi = np.arctan2(w @ t2, w @ t1)
         h, _ = np.histogram(phi, bins=16, range=(-math.pi, math.pi))
         assert np.abs(h / len(w) - 1 / 16).max() < 0.004
+
+
+def test_reflect_mirror(oracle):
+    """Compat mode's mirror rays (reflect, src/util/math.h:231-234): the tangential
+    part is kept, the normal part flips, the length is kept."""
+    rng = np.random.default_rng(14)
+    for n in _normals(rng, 300):
+        i = rng.normal(size=3).astype(np.float32)
+        r = oracle.reflect(i, n).astype(np.float64)
+        i64, n64 = i.astype(np.float64), n.astype(np.float64)
+        assert r @ n64 == pytest.approx(-(i64 @ n64), abs=2e-6)
+        np.testing.assert_allclose(r - (r @ n64) * n64, i64 - (i64 @ n64) * n64, atol=3e-6)
+        assert np.linalg.norm(r) == pytest.approx(np.linalg.norm(i64), rel=2e-6)
