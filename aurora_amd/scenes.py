@@ -632,6 +632,41 @@ def furnace(albedo: float = 0.5, emission: float = 1.0, subdivisions: int = 2, w
                    spp=4, max_depth=max_depth, name="furnace")
 
 
+def surface(albedo, emission=(0.0, 0.0, 0.0), reflectivity: float = 0.0, translucence: float = 0.0,
+            refractive_index: float = 1.0, diffuse: float = 1.0, texture: int = 0) -> np.void:
+    """Shader record for the analytic scenes: Lambert albedo `albedo`
+    (diffuse_color, scaled by `diffuse`), emission colour, and compat mode's
+    mirror weight (reflectivity) and refraction weight (translucence)."""
+    s = _lambert(albedo, emission)
+    s["diffuse"] = diffuse
+    s["reflectivity"] = reflectivity
+    s["translucence"] = translucence
+    s["refractive_index"] = refractive_index
+    s["texture"] = texture
+    return s
+
+
+def horizontal(y: float, up: bool, half: float = 128.0, flip_normals: bool = False):
+    """Square of side 2*half in the plane at height y, centred on the y axis,
+    facing +y (up) or -y.  Corners are integers and the edges powers of two,
+    so hit heights on it are exact up to the ray's own rounding.  With
+    flip_normals the vertex (shading) normals point away from the face."""
+    s = 2.0 * half
+    if up:
+        tris, nrm = _quad((-half, y, -half), (0.0, 0.0, s), (s, 0.0, 0.0))
+    else:
+        tris, nrm = _quad((-half, y, -half), (s, 0.0, 0.0), (0.0, 0.0, s))
+    return tris, (-nrm if flip_normals else nrm)
+
+
+def planes(parts, shaders, lights, camera, width: int = 24, height: int = 16, spp: int = 16,
+           max_depth: int = 1, name: str = "planes") -> Scene:
+    """Scene of the given (triangles, normals) parts; part k uses shader k."""
+    ids = [np.full(p[0].shape[0], k + 1) for k, p in enumerate(parts)]
+    return _finish(list(parts), ids, list(shaders), list(lights), camera, width=width, height=height,
+                   spp=spp, max_depth=max_depth, name=name)
+
+
 # --------------------------------------------------------------------------
 # ray sets: records of 8 float32 (ox oy oz tmax dx dy dz 0)
 
