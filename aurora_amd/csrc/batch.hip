@@ -138,6 +138,7 @@ aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* job
   }
   run.jobs = dev.data();
   run.njobs = njobs;
+  run.clear_images = true;
   run.begin = 0;
   run.end = start;
   run.cap = (uint32_t)std::min<unsigned long long>(std::max<uint32_t>(ctx->render_params.max_paths, 1u), start);

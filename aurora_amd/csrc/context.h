@@ -280,6 +280,7 @@ struct PbrRun {
   unsigned long long begin = 0, end = 0;
   uint32_t cap = 0, levels = 0, nslots = 0, max_depth = 0;
   std::vector<float>* pass_end_ms = nullptr;
+  bool clear_images = false;  // zero every job's image before the first pass (batches)
 };
 aur_status pbr_run(aur_context* ctx, const PbrRun& run, unsigned long long* stats);
 }  // namespace aur

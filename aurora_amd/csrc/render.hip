@@ -805,6 +805,12 @@ __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
   if (threadIdx.x == 0) *p = v;
 }
 
+// Clears one accumulation buffer of n pixels.
+__global__ void k_clear_pixels(float4* __restrict__ px, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) px[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // Clears each job's accumulation buffer (block b: jobs b, b + grid, ...).
 __global__ void k_clear_jobs(const JobDev* __restrict__ jobs, uint32_t njobs) {
   for (uint32_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
@@ -1086,8 +1092,14 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     AUR_HIP(ctx, hipStreamSynchronize(s));  // run.jobs is pageable host memory
     a.jobs = fr.jobs.p;
     a.path_job = fr.path_job.p;
-    // a batch renders its jobs whole: clear their accumulation buffers
-    k_clear_jobs<<<std::min<uint32_t>(run.njobs, 1024), 256, 0, s>>>(fr.jobs.p, run.njobs);
+  }
+  // a batch renders its jobs whole: clear their accumulation buffers first
+  // (one job or many; frames accumulate onto the existing buffer instead)
+  if (run.clear_images) {
+    if (kMulti)
+      k_clear_jobs<<<std::min<uint32_t>(run.njobs, 1024), 256, 0, s>>>(fr.jobs.p, run.njobs);
+    else
+      k_clear_pixels<<<nblocks(a.job.npix, 256), 256, 0, s>>>(a.job.pixels, a.job.npix);
   }
   a.g0p = fr.pass_g0.p;
   a.end = run.end;

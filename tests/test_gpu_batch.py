@@ -111,3 +111,25 @@ def test_frame_graph_equals_direct(gpu_ctx):
         if img0 is None:
             img0 = img
         assert np.array_equal(img, img0)
+
+
+@pytest.mark.parametrize("job", [1, 3])
+def test_one_job_batch_clears_its_image(batch_ctx, reference_images, job):
+    """A batch of one job renders its image whole: into a NaN-filled buffer,
+    twice in a row into the same buffer, with and without graph replay, it
+    equals the single-frame render bit for bit."""
+    import torch
+
+    ctx, jobs = batch_ctx
+    one = jobs[job:job + 1].copy()
+    one["out_offset"] = 0
+    n = int(one[0]["width"]) * int(one[0]["height"])
+    out = torch.full((n, 4), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for max_paths, graph in ((0, 0), (0, 0), (n, 1), (n, 1)):
+        ctx.set_render_params(max_paths, graph)
+        try:
+            ctx.batch_render_ptr(one, out.data_ptr())
+        finally:
+            ctx.set_render_params(0, 0)
+        assert np.array_equal(out.cpu().numpy(), reference_images[job])
