@@ -177,17 +177,19 @@ aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* ve
 aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader* shaders) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (count == 0 || !shaders) return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: empty");
-  bool textured = false;
+  bool textured = false, branching = false;
   for (uint32_t i = 0; i < count; ++i) {
     if (shaders[i].type != 0 && shaders[i].type != 1)
       return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: unknown type");
     textured = textured || shaders[i].texture != 0;
+    branching = branching || (shaders[i].reflectivity > 0.0f && shaders[i].translucence > 0.0f);
   }
   AUR_HIP(ctx, hipSetDevice(ctx->device));
   AUR_HIP(ctx, ctx->shaders.reserve(count));
   AUR_HIP(ctx, hipMemcpy(ctx->shaders.p, shaders, count * sizeof(aur_shader), hipMemcpyHostToDevice));
   ctx->num_shaders = count;
   ctx->textured_shaders = textured;
+  ctx->branching_shaders = branching;
   return AUR_OK;
 }
 
@@ -232,8 +234,7 @@ aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* 
     AUR_HIP(ctx, hipMemcpy(ctx->lights.p, lights, count * sizeof(aur_light), hipMemcpyHostToDevice));
   }
   ctx->num_lights = count;
-  ctx->num_slots = 0;
-  for (uint32_t i = 0; i < count; ++i) ctx->num_slots += aur::light_slots(lights[i]);
+  ctx->num_slots = aur::light_set_slots(count, lights);
   return AUR_OK;
 }
 

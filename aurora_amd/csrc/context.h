@@ -26,6 +26,14 @@ __host__ __device__ inline uint32_t light_slots(const aur_light& L) {
   return L.type == AUR_LIGHT_AMBIENT ? 2u : 1u;
 }
 
+// Slots of a light set, summed in 64 bits and saturated at 2^32 - 1: any sum
+// above kMaxSlots makes PBR renders refuse the set (compat mode has no limit).
+inline uint32_t light_set_slots(uint32_t count, const aur_light* lights) {
+  uint64_t sum = 0;
+  for (uint32_t i = 0; i < count; ++i) sum += light_slots(lights[i]);
+  return sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)sum;
+}
+
 // The device allocator in force (aur_set_alloc_hook); alloc == nullptr means
 // hipMalloc / hipFree.
 aur_alloc_hook current_alloc_hook();
@@ -144,6 +152,7 @@ struct FrameState {
   DevBuf<int32_t> sh_occ;
   DevBuf<uint32_t> sh_queue;  // slots holding a shadow ray, appended by shading
   DevBuf<float2> gstack;      // traversal stack rows beyond the LDS window
+  DevBuf<float4> compat_pend; // compat mode: mirror rays waiting below a refraction ray
   DevBuf<uint32_t> sh_job;    // batches: job of each shadow-queue entry
   // adjoint replay: path-vertex records, per-block partial gradients,
   // running gradient sums and the largest |dLoss/dI|
@@ -199,6 +208,7 @@ struct aur_context {
   uint32_t num_lights = 0;
   uint32_t num_slots = 0;  // next-event slots per path vertex
   bool textured_shaders = false;  // a shader uses a texture map
+  bool branching_shaders = false;  // a shader is both reflective and translucent (two compat child rays)
   // RGBA8 textures (aur_textures_upload)
   aur::DevBuf<aur::TexDesc> textures;
   aur::DevBuf<uint8_t> texels;

@@ -124,13 +124,25 @@ __device__ inline bool occluded(const SceneDev& sc, V3 o, V3 d, float tmax, floa
   return trace_ray<true>(sc.nodes, sc.n_nodes, sc.rec_bytes, o, d, tmax, stk, stride).id >= 0;
 }
 
+// The reference's ray stack (Stack<Ray, MAX_DEPTH + 2>, popped last-in
+// first-out) without a dynamically indexed per-lane array: the ray on top of
+// the stack stays in registers, and a hit pushes a ray into memory only when
+// it spawns two (mirror and refraction): the mirror ray, pushed first, waits
+// in `pend` below the refraction ray, which becomes the top.  Each such hit
+// raises depth, and a child is pushed only while depth <= 8, so at most
+// kCompatPend rays wait.  pend rows are [row][lane] (two float4 per entry:
+// origin + weight, direction + eta), `pend_stride` lanes apart; null when no
+// shader is both reflective and translucent.
+constexpr int kCompatPend = 8;
+
 // region: (left, bottom, region width, region pixels); thread l renders
 // region pixel l, frame pixel (bottom + l / width) * w + left + l % width.
 __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, uint32_t w,
                                                    uint32_t h, uint4 region, uint32_t spp,
                                                    uint32_t sample, uint32_t seed,
                                                    float4* __restrict__ pixels,
-                                                   unsigned long long* __restrict__ nrays_total) {
+                                                   unsigned long long* __restrict__ nrays_total,
+                                                   float4* __restrict__ pend, uint32_t pend_stride) {
   extern __shared__ float2 stk_mem[];
   float2* stk = stk_mem + threadIdx.x;
   const uint32_t stride = blockDim.x;
@@ -141,20 +153,32 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
     const uint32_t pix = y * w + x;
     CompatRng rng{seed, pix, sample, 0u, U4{0, 0, 0, 0}};
     const int kMaxDepth = 8;
-    CRay stack[kMaxDepth + 2];
-    int ns = 0;
-    stack[ns++] = CRay{ld3(cam.position), camera_dir_compat(cam, w, h, x, y, sample, spp), kInfinity,
-                       1.0f, 1.0f};
+    float4* mine = pend + 2 * (size_t)l;
+    const size_t row = 2 * (size_t)pend_stride;
+    int np = 0;  // rays waiting in pend below the top
+    CRay top{ld3(cam.position), camera_dir_compat(cam, w, h, x, y, sample, spp), kInfinity, 1.0f, 1.0f};
+    bool have = true;  // the stack holds `top` (and the np rays below it)
+    auto pop_pending = [&]() {
+      if (np == 0) return false;
+      --np;
+      const float4 a = mine[np * row], b = mine[np * row + 1];
+      top = CRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), kInfinity, a.w, b.w};
+      return true;
+    };
     int depth = 0;
     V3 color = mk(0.0f);
     int32_t hpId = -1;
     float hpU = 0.0f, hpV = 0.0f;
-    while (depth <= kMaxDepth && ns > 0) {
-      CRay ray = stack[--ns];
+    while (depth <= kMaxDepth && have) {
+      CRay ray = top;
+      have = false;
       ++nrays;
       const TraceResult hr =
           trace_ray<false>(sc.nodes, sc.n_nodes, sc.rec_bytes, ray.pos, ray.dir, ray.t, stk, stride);
-      if (hr.id < 0) continue;
+      if (hr.id < 0) {
+        have = pop_pending();
+        continue;
+      }
       ray.t = hr.t;
       hpId = hr.id;
       hpU = hr.u;
@@ -230,9 +254,9 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
       }
       if (sh.reflectivity > 0.0f || sh.translucence > 0.0f) depth++;
       if (depth <= kMaxDepth && sh.reflectivity > 0.0f) {
-        CRay nr{position, reflect(ray.dir, N), kInfinity, sh.reflectivity, ray.eta};
+        top = CRay{position, reflect(ray.dir, N), kInfinity, sh.reflectivity, ray.eta};
         ray.weight *= (1.0f - sh.reflectivity);
-        stack[ns++] = nr;
+        have = true;
       }
       if (depth <= kMaxDepth && sh.translucence > 0.0f) {
         V3 rN = N;
@@ -244,11 +268,17 @@ __global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, 
         }
         V3 tdir;
         if (refract(tdir, ray.dir, rN, etai, etat)) {
-          CRay nr{position, tdir, kInfinity, sh.translucence, etat};
+          if (have) {  // the mirror ray waits below the refraction ray
+            mine[np * row] = make_float4(top.pos.x, top.pos.y, top.pos.z, top.weight);
+            mine[np * row + 1] = make_float4(top.dir.x, top.dir.y, top.dir.z, top.eta);
+            ++np;
+          }
+          top = CRay{position, tdir, kInfinity, sh.translucence, etat};
           ray.weight *= (1.0f - sh.translucence);
-          stack[ns++] = nr;
+          have = true;
         }
       }
+      if (!have) have = pop_pending();
       color = color + (Li * ray.weight);
     }
     if (hpId != -1) {
@@ -1338,6 +1368,14 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
                 ctx->num_textures};
     const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
     const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
+    // waiting mirror rays exist only under shaders that are both reflective
+    // and translucent
+    const uint32_t lanes = nblocks(npix, kBlock) * kBlock;
+    float4* pend = nullptr;
+    if (ctx->branching_shaders) {
+      AUR_HIP(ctx, fr.compat_pend.reserve((size_t)kCompatPend * 2 * lanes));
+      pend = fr.compat_pend.p;
+    }
     PassThrottle throttle(s);
     for (uint32_t smp = s0; smp < s1; ++smp) {
       AUR_HIP(ctx, throttle.wait(smp - s0));
@@ -1346,7 +1384,7 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
         return ctx->fail(AUR_ERR_CANCELLED, "render cancelled");
       }
       k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, region, o->spp, smp, o->seed,
-                                                          fr.pixels.p, stats);
+                                                          fr.pixels.p, stats, pend, lanes);
       AUR_LAUNCH_CHECK(ctx, "k_compat");
       AUR_HIP(ctx, throttle.mark(smp - s0));
     }

@@ -127,34 +127,40 @@ extern "C" aur_status aur_geometry_upload_objects(aur_context* ctx, uint32_t cou
     std::memcpy(&uv[(size_t)count * 6], &uv[(size_t)(count - 1) * 6], 6 * sizeof(float));
     ids[count] = ids[count - 1];
   }
-  ctx->built = false;
-  ctx->has_tb = false;
-  ctx->count = 0;
-  DevBuf<float> sv, sn;
+  // transform into staging buffers; the context's geometry (and a built
+  // tree) changes only once the transformed vertices are known to be finite
+  DevBuf<float> sv, sn, tv, tn, tuv;
+  DevBuf<uint16_t> tids;
   DevBuf<aur_transform> dxf;
   DevBuf<uint32_t> bad;
   AUR_HIP(ctx, sv.reserve((size_t)N * 9));
   AUR_HIP(ctx, sn.reserve((size_t)N * 9));
   AUR_HIP(ctx, bad.reserve(1));
   AUR_HIP(ctx, dxf.reserve(std::max<uint32_t>(num_objects, 1)));
-  AUR_HIP(ctx, ctx->in_vertices.reserve((size_t)N * 9));
-  AUR_HIP(ctx, ctx->in_normals.reserve((size_t)N * 9));
-  AUR_HIP(ctx, ctx->in_texcoords.reserve((size_t)N * 6));
-  AUR_HIP(ctx, ctx->in_shader_ids.reserve(N));
+  AUR_HIP(ctx, tv.reserve((size_t)N * 9));
+  AUR_HIP(ctx, tn.reserve((size_t)N * 9));
+  AUR_HIP(ctx, tuv.reserve((size_t)N * 6));
+  AUR_HIP(ctx, tids.reserve(N));
   AUR_HIP(ctx, hipMemcpy(sv.p, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
   AUR_HIP(ctx, hipMemcpy(sn.p, nrm.data(), nrm.size() * sizeof(float), hipMemcpyHostToDevice));
   if (num_objects)
     AUR_HIP(ctx, hipMemcpy(dxf.p, transforms, num_objects * sizeof(aur_transform), hipMemcpyHostToDevice));
-  AUR_HIP(ctx, hipMemcpy(ctx->in_texcoords.p, uv.data(), uv.size() * sizeof(float), hipMemcpyHostToDevice));
-  AUR_HIP(ctx, hipMemcpy(ctx->in_shader_ids.p, ids.data(), N * sizeof(uint16_t), hipMemcpyHostToDevice));
+  AUR_HIP(ctx, hipMemcpy(tuv.p, uv.data(), uv.size() * sizeof(float), hipMemcpyHostToDevice));
+  AUR_HIP(ctx, hipMemcpy(tids.p, ids.data(), N * sizeof(uint16_t), hipMemcpyHostToDevice));
   AUR_HIP(ctx, hipMemsetAsync(bad.p, 0, sizeof(uint32_t), ctx->stream));
   aur::k_transform<<<(N + 255) / 256, 256, 0, ctx->stream>>>(sv.p, sn.p, N, dxf.p, num_objects, ascending,
-                                                              ctx->in_vertices.p, ctx->in_normals.p, bad.p);
+                                                              tv.p, tn.p, bad.p);
   AUR_LAUNCH_CHECK(ctx, "k_transform");
   uint32_t hbad = 0;
   AUR_HIP(ctx, hipMemcpyAsync(&hbad, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   AUR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (hbad) return ctx->fail(AUR_ERR_INVALID_ARG, "objects: a transformed vertex is not finite");
+  ctx->in_vertices.swap(tv);
+  ctx->in_normals.swap(tn);
+  ctx->in_texcoords.swap(tuv);
+  ctx->in_shader_ids.swap(tids);
+  ctx->built = false;
+  ctx->has_tb = false;
   ctx->count = N;
   return AUR_OK;
 }

@@ -77,6 +77,18 @@ def test_geometry_rejections_keep_the_built_tree(fresh, oracle):
     st = fresh._lib.aur_geometry_upload(fresh.handle, 4, None, None, None, None)
     assert st == INVALID_ARG  # null vertices
 
+    # object-space uploads: a transform whose product overflows (finite
+    # entries, infinite transformed vertex) is found on the device, after
+    # the upload was staged; the built tree stays
+    big = np.eye(4) * 1e30
+    big[3, 3] = 1.0
+    xf = np.array([scenes.make_transform(0, 3000, big)], dtype=scenes.TRANSFORM_DTYPE)
+    assert _status(fresh.upload_objects, v * np.float32(1e10), s.normals, xf) == INVALID_ARG
+    w = v.copy()
+    w[7, 0] = np.nan
+    ok = np.array([scenes.make_transform(0, 3000, np.eye(4))], dtype=scenes.TRANSFORM_DTYPE)
+    assert _status(fresh.upload_objects, w, s.normals, ok) == INVALID_ARG
+
     # the tree built before the rejected uploads is still there and traces
     assert np.array_equal(fresh.read_permutation(), perm)
     rays = scenes.incoherent_rays(20_000, 0.0, 10.0, seed=22)
@@ -221,3 +233,30 @@ def test_next_event_slot_limit(fresh):
     out = torch.zeros((64, 4), dtype=torch.float32, device="cuda")
     assert _status(fresh.batch_render_ptr, jobs, out.data_ptr()) == UNSUPPORTED
     assert float(out.abs().sum()) == 0.0
+
+
+def test_slot_sum_does_not_wrap(fresh):
+    """131,076 area lights of 32,767 samples and one of 68 take 2^32 + 64
+    next-event slots: the count saturates instead of wrapping to 64, so PBR
+    renders and batches refuse the set before any kernel runs."""
+    import torch
+
+    s = scenes.cornell(width=8, height=8, subdivisions=1)
+    fresh.upload_scene(s)
+    fresh.rebuild_nmh()
+    fresh.frame_create(8, 8, 1)
+    L = scenes.area_light((2.0, 4.9, 2.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, -1.0, 0.0), (1.0, 1.0, 1.0),
+                          samples=32767)
+    lights = np.array([L] * 131077, dtype=scenes.LIGHT_DTYPE)
+    lights[-1]["samples"] = 68
+    assert 131076 * 32767 + 68 == (1 << 32) + 64
+    fresh.upload_lights(lights)
+    pbr = native.Context.opts(native.MODE_PBR, 1)
+    assert _status(fresh.frame_render, s.camera, pbr) == UNSUPPORTED
+    assert float(np.abs(fresh.frame_read()).sum()) == 0.0
+    s.lights = lights
+    sid = fresh.scene_add(s)
+    jobs = native.Context.make_jobs([dict(scene=sid, width=8, height=8, spp=1, max_depth=2, rr_start=2,
+                                          seed=1, camera=s.camera)])
+    out = torch.zeros((64, 4), dtype=torch.float32, device="cuda")
+    assert _status(fresh.batch_render_ptr, jobs, out.data_ptr()) == UNSUPPORTED
