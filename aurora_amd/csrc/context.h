@@ -216,6 +216,11 @@ struct aur_context {
 
   aur::FrameState frame;
 
+  // light sampling tables: scene bounds (min xyz, max xyz), per-block bounds
+  // partials, and the emitter table's per-tile counts and emission sums
+  aur::DevBuf<float> bounds, bounds_partial;
+  aur::DevBuf<double> emit_scratch;
+
   // built scenes kept for batches (aur_scene_add)
   std::vector<std::unique_ptr<aur::SceneStore>> scenes;
 

@@ -69,6 +69,11 @@ JOB_DTYPE = np.dtype({
     "itemsize": 104,
 })
 
+# aur_emitter (include/aurora_amd.h): the reference's Emitter, 28 B
+EMITTER_DTYPE = np.dtype([("area", "<f4"), ("pdf", "<f4"), ("cdf", "<f4"), ("power", "<f4", 3),
+                          ("triangle", "<u4")])
+assert EMITTER_DTYPE.itemsize == 28
+
 # aur_transform (include/aurora_amd.h): the reference's Transform, 136 B
 TRANSFORM_DTYPE = scenes.TRANSFORM_DTYPE
 
@@ -149,6 +154,9 @@ def load():
         "aur_scene_count": ([P], u32),
         "aur_batch_render": ([P, u32, P, P, P], i32),
         "aur_frame_adjoint": ([P, P, ctypes.POINTER(RenderOpts), P, P, P], i32),
+        "aur_scene_bounds": ([P, P, P], i32),
+        "aur_create_lights_cdf": ([P, P, ctypes.POINTER(u32)], i32),
+        "aur_create_emitters": ([P, P, u64, ctypes.POINTER(u32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -210,6 +218,7 @@ class Context:
             raise AuroraError(st, f"aur_create(device={device}) failed")
         self._h = h
         self.count = 0
+        self.num_lights = 0
 
     # -- plumbing --
     def _check(self, st: int, what: str) -> None:
@@ -291,6 +300,7 @@ class Context:
         li = np.ascontiguousarray(lights, dtype=scenes.LIGHT_DTYPE)
         self._check(self._lib.aur_lights_upload(self._h, li.shape[0], _ptr(li) if li.shape[0] else None),
                     "lights upload")
+        self.num_lights = li.shape[0]
 
     def upload_scene(self, scene: scenes.Scene) -> None:
         self.upload_geometry(scene.vertices, scene.normals, scene.texcoords, scene.shader_ids)
@@ -324,6 +334,44 @@ class Context:
             "vertices", "normals", "texcoords", "shader_ids", "tangents", "bitangents"))),
             "read_geometry")
         return out
+
+    # -- light sampling tables --
+    def scene_bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        self._check(self._lib.aur_scene_bounds(self._h, _ptr(lo), _ptr(hi)), "scene_bounds")
+        return lo, hi
+
+    def lights_cdf_ptr(self, cdf_ptr: int) -> int:
+        """Writes the light-power CDF to the device buffer; returns the count."""
+        n = ctypes.c_uint32()
+        self._check(self._lib.aur_create_lights_cdf(self._h, ctypes.c_void_p(cdf_ptr), ctypes.byref(n)),
+                    "create_lights_cdf")
+        return int(n.value)
+
+    def emitters_ptr(self, out_ptr: int, capacity: int) -> int:
+        """Writes the emitter table (EMITTER_DTYPE records) to the device
+        buffer; returns the emitter count."""
+        n = ctypes.c_uint32()
+        self._check(self._lib.aur_create_emitters(self._h, ctypes.c_void_p(out_ptr), capacity, ctypes.byref(n)),
+                    "create_emitters")
+        return int(n.value)
+
+    def lights_cdf(self) -> np.ndarray:
+        import torch
+
+        buf = torch.zeros(max(1, self.num_lights), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        n = self.lights_cdf_ptr(buf.data_ptr())
+        return buf[:n].cpu().numpy()
+
+    def emitters(self, capacity: int | None = None) -> np.ndarray:
+        import torch
+
+        cap = self.count if capacity is None else capacity
+        buf = torch.zeros(max(1, cap) * EMITTER_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        n = self.emitters_ptr(buf.data_ptr(), cap)
+        return buf[: n * EMITTER_DTYPE.itemsize].cpu().numpy().view(EMITTER_DTYPE)
 
     # -- traversal (device pointers) --
     def trace_closest_ptr(self, n: int, rays_ptr: int, hits_ptr: int) -> None:
@@ -462,6 +510,7 @@ class Context:
                                             _ptr(sh), li.shape[0], _ptr(li) if li.shape[0] else None,
                                             ctypes.byref(out)), "scene_add")
         self.count = 0
+        self.num_lights = 0
         return int(out.value)
 
     def scene_clear(self) -> None:

@@ -7,18 +7,31 @@ directions uniform on the sphere.  A step is one launch over the 2^24 rays,
 already resident in HBM.  value = rays traced per second (Mrays/s), timed
 with HIP events on the library's stream.
 
-Roofline: algorithmic bytes per ray B = 32 + 16 + 24 P + 48 S (ray in, hit
-out, two triangles' axis coordinates for every popped in-range node P, their
-other twelve floats for every slab-passing node S), with P and S counted by
-an instrumented launch over the same rays.  achieved = sum(B) / mean launch
-time; peak = 8 TB/s HBM; traffic = calibrated rocprofv3 FETCH_SIZE +
-WRITE_SIZE bytes per launch from the committed summary.
+Roofline: the traversal kernel is bound by VALU issue, not by HBM (its node
+records, 49 MB at 1M triangles, are served from L2 and the Infinity Cache).
+roofline.achieved = VALU lane-instructions per second (wave VALU
+instructions per launch from the committed rocprofv3 counter summary of the
+same command, x 64 lanes, over the launch time measured here); peak = 39.3 T
+lane-instructions/s.  Beside it: the HBM model, algorithmic bytes per ray
+B = 32 + 16 + 24 P + 48 S (ray in, hit out, two triangles' axis coordinates
+for every popped in-range node P, their other twelve floats for every
+slab-passing node S; P and S counted by an instrumented launch over the same
+rays) against 8 TB/s, and traffic = calibrated FETCH_SIZE + WRITE_SIZE bytes
+per launch.
 
-cpu_baseline: the CPU restatement of the reference traversal (the oracle)
-on a bounded sample of the same rays, all host threads.
+cpu_baseline: the CPU restatement of the reference traversal (the oracle),
+built with -O3 -march=native on the host that runs it, on a bounded sample of
+the same rays with OpenMP on every host thread; its hits must equal the GPU's.
 
 Other workloads print their own JSON line:
-  soup10m  10M-triangle soup: NMH build ms, incoherent and coherent Mrays/s
+  soup10m          10M-triangle soup: NMH build ms, incoherent and coherent Mrays/s
+  render_cornell   PBR seconds to 1% rel-RMSE, Cornell box 256^2
+  render_spheres   PBR seconds to 1% rel-RMSE, 1M triangles 1080p
+  compat_cornell   compat-mode (reference MC ray tracer) frame, 256^2 16 spp
+  compat_spheres   compat-mode frame, 1M triangles, 1080p 64 spp
+  batch256         256 batched jobs vs per-job frames
+  adjoint_cornell  adjoint/forward time, finite differences, Adam fit
+  emitters         emitter table (compaction + prefix sums) over 1M triangles
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload trace1m]
 """
@@ -36,45 +49,69 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
+# = 39.3 T lane-instructions/s (the 157.3 TFLOP/s FP32 vector peak counts each
+# lane-instruction as a fused multiply-add on a packed pair: x 2 x 2)
+VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def parse_traffic(kernel: str, summary: str):
-    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) from the
-    committed rocprofv3 summary's 'traffic_bytes_per_launch <kernel> <B>' line."""
+def parse_profile(key: str, kernel: str, summary: str):
+    """A per-launch figure of a kernel from the committed rocprofv3 summary's
+    machine-readable '<key> <kernel> <value>' lines: traffic_bytes_per_launch
+    (calibrated FETCH_SIZE + WRITE_SIZE) or valu_insts_per_launch
+    (SQ_INSTS_VALU, wave instructions)."""
     path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-            return int(f[2])
+        if len(f) == 3 and f[0] == key and f[1] == kernel:
+            return int(float(f[2]))
     return None
 
 
-def cpu_trace_baseline(tree_vertices: np.ndarray, rays: np.ndarray, budget_s: float = 15.0):
+def parse_traffic(kernel: str, summary: str):
+    return parse_profile("traffic_bytes_per_launch", kernel, summary)
+
+
+def cpu_trace_baseline(tree_vertices: np.ndarray, rays: np.ndarray, gpu_ids: np.ndarray,
+                       gpu_tuv: np.ndarray, budget_s: float = 12.0):
+    """The oracle's closest-hit traversal (the reference's intersect(),
+    restated) built with -O3 -march=native on this host, OpenMP over rays on
+    every host thread: one warm-up, then three timed runs over a sample sized
+    to about budget_s in total (median).  The sample's hit ids and t/u/v
+    must equal the GPU's before the timing counts."""
     from tests import oracle_ffi
 
     threads = os.cpu_count() or 1
-    oracle_ffi.lib()
-    n = 4096
+    oracle_ffi.use_native()
+    n = 8192
     t0 = time.perf_counter()
-    oracle_ffi.trace(tree_vertices, rays[:n], threads=threads)
+    oracle_ffi.trace(tree_vertices, rays[:n], threads=threads)  # warm-up
     dt = time.perf_counter() - t0
-    # grow the sample to roughly budget_s of CPU work
-    n2 = int(min(rays.shape[0], max(n, n * budget_s / max(dt, 1e-3) * 0.5)))
-    t0 = time.perf_counter()
-    oracle_ffi.trace(tree_vertices, rays[:n2], threads=threads)
-    dt = time.perf_counter() - t0
+    n2 = int(min(rays.shape[0], max(n, n * budget_s / 3 / max(dt, 1e-3))))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ids, tuv = oracle_ffi.trace(tree_vertices, rays[:n2], threads=threads)
+        runs.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(ids, gpu_ids[:n2]) and
+                np.array_equal(tuv.view(np.uint32), gpu_tuv[:n2].view(np.uint32)))
+    if not same:
+        raise RuntimeError("cpu baseline: CPU and GPU hits differ on the timed sample")
     return {
-        "value": round(n2 / dt / 1e6, 3),
+        "value": round(n2 / float(np.median(runs)) / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {n2} of the 2^24 rays, closest hit, oracle traversal on {threads} threads",
+        "sample": f"first {n2} of the 2^24 rays, closest hit, median of 3 runs after a warm-up; "
+                  f"hit ids and t/u/v equal the GPU's",
+        "cpu_model": oracle_ffi.cpu_model(),
+        "build": "g++ -O3 -march=native -ffp-contract=off -fopenmp",
     }
 
 
@@ -138,7 +175,8 @@ def workload_trace1m(a) -> dict:
     cpu = None
     if not a.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_trace_baseline(scene.vertices[perm], rays_np)
+        h = hits.cpu().numpy()
+        cpu = cpu_trace_baseline(scene.vertices[perm], rays_np, h[:, 0].view(np.int32), h[:, 1:4])
         log(f"cpu baseline {cpu['value']} Mrays/s on {cpu['cores']} threads")
     ctx.close()
     return {
@@ -165,16 +203,37 @@ def workload_trace1m(a) -> dict:
             "hit_rate": round(hit_rate, 4),
             "trace_params": params,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": parse_traffic("k_trace_pt", "k_trace_incoherent_1M.md"),
-        },
+        "roofline": trace_roofline(bytes_per_launch, ms_step, "k_trace_incoherent_1M.md"),
         "cpu_baseline": cpu,
     }
+
+
+def trace_roofline(model_bytes: int, ms: float, summary: str) -> dict:
+    """The traversal kernel's binding roof is VALU issue: its node records
+    (49 MB at 1M triangles) are served from L2 and the Infinity Cache, so
+    HBM traffic is a fraction of the algorithmic bytes.  achieved = wave
+    VALU instructions per launch (SQ_INSTS_VALU from the committed profile of
+    the same command) x 64 lanes / the launch time measured here; peak =
+    VALU_PEAK_T.  The HBM model (32 + 16 + 24 P + 48 S bytes per ray) and the
+    measured HBM bytes stay beside it."""
+    valu = parse_profile("valu_insts_per_launch", "k_trace_pt", summary)
+    traffic = parse_traffic("k_trace_pt", summary)
+    sec = ms * 1e-3
+    model_gbs = model_bytes / sec / 1e9
+    out = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_T, 2), "unit": "T VALU lane-instr/s",
+           "frac": None, "traffic": traffic,
+           "valu_insts_per_launch": valu,
+           "hbm_model": {"bytes_per_launch": int(model_bytes), "achieved_gbs": round(model_gbs, 1),
+                         "frac_of_8tbs": round(model_gbs / HBM_PEAK_GBS, 4)},
+           "hbm_measured": None}
+    if valu:
+        t = valu * 64 / sec / 1e12
+        out["achieved"] = round(t, 2)
+        out["frac"] = round(t / VALU_PEAK_T, 4)
+    if traffic:
+        g = traffic / sec / 1e9
+        out["hbm_measured"] = {"achieved_gbs": round(g, 1), "frac_of_8tbs": round(g / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def workload_soup10m(a) -> dict:
@@ -189,9 +248,11 @@ def workload_soup10m(a) -> dict:
     ctx.upload_scene(scene)
     builds = [ctx.rebuild_nmh().build_ms for _ in range(3)]
     log(f"NMH build ms: {builds}")
-    out, shapes = {}, {}
-    for name, rays_np in (("incoherent", scenes.incoherent_rays(1 << 24, 0.0, 1000.0, seed=33)),
-                          ("coherent", scenes.coherent_rays(scene.camera, 4096, 4096))):
+    out, shapes, trace_roof = {"incoherent": None, "coherent": None}, {}, {}
+    sets = {"incoherent": lambda: scenes.incoherent_rays(1 << 24, 0.0, 1000.0, seed=33),
+            "coherent": lambda: scenes.coherent_rays(scene.camera, 4096, 4096)}
+    for name in ([] if a.only == "build" else [a.only] if a.only else list(sets)):
+        rays_np = sets[name]()
         rays = torch.from_numpy(rays_np).to("cuda")
         hits = torch.empty((rays.shape[0], 4), dtype=torch.float32, device="cuda")
         n = rays.shape[0]
@@ -202,16 +263,49 @@ def workload_soup10m(a) -> dict:
                               a.steps, a.warmup)
         out[name] = round(n / (ms * 1e-3) / 1e6, 1)
         log(f"{name}: {ms:.3f} ms, {out[name]} Mrays/s (launch shape {shapes[name]})")
+        popped, passed = ctx.trace_closest_stats_ptr(n, rays.data_ptr(), hits.data_ptr())
+        model = 48 * n + 24 * popped + 48 * passed
+        traffic = parse_traffic(f"k_trace_pt:{name}", "soup10m_traversal.md")
+        trace_roof[name] = {"ms": round(ms, 4), "popped_per_ray": round(popped / n, 3),
+                            "passed_per_ray": round(passed / n, 3), "hbm_model_bytes": int(model),
+                            "hbm_model_frac": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "traffic": traffic,
+                            "hbm_measured_frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
+        log(f"{name}: P={popped / n:.1f} S={passed / n:.1f}, model {model / 1e9:.1f} GB per launch")
+        del rays, hits
     cpu = None
     if not a.no_cpu:
         from tests import oracle_ffi
 
-        sub = scene.vertices[: 1 << 21]
-        t0 = time.perf_counter()
-        oracle_ffi.nmh_build(sub, threads=1)
-        cpu = {"value": round((time.perf_counter() - t0) * 1e3, 1), "unit": "ms", "cores": 1,
-               "kind": "port",
-               "sample": "literal NMH build of the first 2,097,152 soup triangles, one core"}
+        # the literal restatement of cudaRebuildNMH (stable sorts per level)
+        # over all 10M triangles, one thread and then OpenMP-parallel sorts on
+        # every host thread; both permutations must equal the GPU's
+        oracle_ffi.use_native()
+        gpu_perm = ctx.read_permutation()
+        threads = os.cpu_count() or 1
+        # one thread: all 10M with --cpu-full, else the first 2^21 triangles
+        # (a build of its own, checked against a GPU build of the same subset)
+        n1 = scene.count if a.cpu_full else (1 << 21)
+        sub_perm = gpu_perm
+        if n1 != scene.count:
+            ctx.upload_geometry(scene.vertices[:n1], scene.normals[:n1], scene.texcoords[:n1],
+                                scene.shader_ids[:n1])
+            ctx.rebuild_nmh()
+            sub_perm = ctx.read_permutation()
+        cpu_ms = {}
+        for th, n, ref in ((1, n1, sub_perm), (threads, scene.count, gpu_perm)):
+            t0 = time.perf_counter()
+            p = oracle_ffi.nmh_build(scene.vertices[:n], threads=th)
+            cpu_ms[th] = (time.perf_counter() - t0) * 1e3
+            if not np.array_equal(p, ref):
+                raise RuntimeError(f"cpu baseline: the {th}-thread CPU tree differs from the GPU tree")
+            log(f"cpu NMH build of {n} triangles, {th} thread(s): {cpu_ms[th]:.0f} ms (tree equals the GPU's)")
+        cpu = {"value": round(cpu_ms[threads], 1), "unit": "ms", "cores": threads, "kind": "port",
+               "single_thread_ms": round(cpu_ms[1], 1), "single_thread_triangles": n1,
+               "sample": f"literal NMH build of all 10,000,000 soup triangles on {threads} threads, and of "
+                         f"the first {n1} on one thread (one run each); permutations equal the GPU's",
+               "cpu_model": oracle_ffi.cpu_model(),
+               "build": "g++ -O3 -march=native -ffp-contract=off -fopenmp"}
     ctx.close()
     build_ms = float(np.median(builds[1:]))
     nb = nmh_build_bytes(scene.count)
@@ -222,10 +316,11 @@ def workload_soup10m(a) -> dict:
         "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded 10M-triangle soup)",
         "config": {"workload": "10M-triangle soup build + 2^24 incoherent + 4096^2 coherent rays",
                    "triangles": scene.count, "incoherent_mrays": out["incoherent"],
-                   "coherent_mrays": out["coherent"], "trace_params": shapes,
+                   "coherent_mrays": out["coherent"], "trace_params": shapes, "traversal_roofline": trace_roof,
                    "build_ms_all": [round(b, 3) for b in builds]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": parse_traffic("nmh_build", "nmh_build_10M.md"),
                      "algorithmic_bytes": nb},
         "cpu_baseline": cpu,
     }
@@ -617,7 +712,109 @@ def workload_batch256(a) -> dict:
     }
 
 
+def compat_frame(a, scene, label: str) -> dict:
+    """Compat mode (the reference's MonteCarloRaytracer::render: 4x4-strata
+    camera rays, estimators F1-F3, mirror and refraction rays) at the
+    scene's spp: wall time of the render call, rays traced, and the two-seed
+    rel-RMSE of the image at that spp."""
+    from aurora_amd import native
+
+    ctx = native.Context(0)
+    ctx.upload_scene(scene)
+    ctx.rebuild_nmh()
+    w, h, spp = scene.width, scene.height, a.spp or scene.spp
+    ctx.frame_create(w, h, spp)
+    o = lambda seed: native.Context.opts(native.MODE_COMPAT, spp, scene.max_depth, scene.rr_start, seed)  # noqa: E731
+    ctx.frame_render(scene.camera, o(1))  # warm-up
+    ctx.synchronize()
+    times = []
+    for _ in range(max(1, a.steps // 4)):
+        t0 = time.perf_counter()
+        ctx.frame_render(scene.camera, o(7))
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    img_a = ctx.frame_read()
+    rays = ctx.rays_traced()
+    ctx.frame_render(scene.camera, o(8))
+    img_b = ctx.frame_read()
+    err = two_seed_rel_rmse(img_a, img_b)
+    ctx.close()
+    t = float(np.median(times))
+    log(f"{label}: compat {w}x{h} {spp} spp: {t * 1e3:.1f} ms, {rays / t / 1e6:.0f} Mrays/s, rel-RMSE {err:.4f}")
+    return {"ms": round(t * 1e3, 3), "mrays": round(rays / t / 1e6, 1), "rays": int(rays), "spp": spp,
+            "rel_rmse_two_seed": round(err, 5), "runs": len(times)}
+
+
+def workload_compat_cornell(a) -> dict:
+    from aurora_amd import scenes
+
+    scene = scenes.cornell()  # 256^2, 16 spp
+    r = compat_frame(a, scene, "cornell")
+    return {"metric": "compat render ms (Cornell box + 2 spheres, 256x256, 16 spp)", "value": r["ms"],
+            "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+            "data": "synthetic (seeded Cornell box + 2 icospheres)",
+            "config": {"workload": "compat mode (reference MC ray tracer), Cornell box + 2 subdivision-3 spheres (2,584 triangles), 256x256, 16 spp", **r}}
+
+
+def workload_compat_spheres(a) -> dict:
+    from aurora_amd import scenes
+
+    scene = scenes.spheres()  # 1920x1080, 64 spp, Phong materials with reflectivity 0.2
+    r = compat_frame(a, scene, "spheres")
+    return {"metric": "compat render ms (200 spheres, 1M triangles, 1920x1080, 64 spp)", "value": r["ms"],
+            "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+            "data": "synthetic (seeded 200 icospheres + ground, 4 quad lights)",
+            "config": {"workload": "compat mode (reference MC ray tracer), 200 subdivision-4 spheres + ground (1,024,002 triangles), 4 quad lights, 1920x1080, 64 spp", **r}}
+
+
+def emitter_bytes(n: int, emitters: int) -> int:
+    """Algorithmic bytes of one emitter-table build: the shader ids read by
+    the counting and the writing pass (2 B each per triangle), and per
+    emitter its vertices (36 B) and its record (28 B)."""
+    return 4 * n + 64 * emitters
+
+
+def workload_emitters(a) -> dict:
+    """aur_create_emitters on the 1M-triangle sphere scene with every other
+    material emissive (cudaCreateEmitters, src/kernels/emitter.cu:79-119)."""
+    import torch
+
+    from aurora_amd import native, scenes
+
+    scene = scenes.spheres()
+    scene.shaders["emission"][::2] = np.linspace(0.5, 2.0, 16, dtype=np.float32)
+    ctx = native.Context(0)
+    ctx.upload_scene(scene)
+    ctx.rebuild_nmh()
+    n = scene.count
+    buf = torch.empty(n * 28, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    count = ctx.emitters_ptr(buf.data_ptr(), n)
+    times = []
+    for i in range(a.warmup + a.steps):
+        t0 = time.perf_counter()
+        ctx.emitters_ptr(buf.data_ptr(), n)
+        if i >= a.warmup:
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    nb = emitter_bytes(n, count)
+    cdf = torch.empty(max(1, ctx.num_lights), dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    ctx.lights_cdf_ptr(cdf.data_ptr())
+    t_cdf = time.perf_counter() - t0
+    ctx.close()
+    log(f"emitters: {count} of {n} triangles, {t * 1e3:.3f} ms per call (host wall), lights cdf {t_cdf * 1e3:.3f} ms")
+    return {"metric": "emitter table ms (1M triangles)", "value": round(t * 1e3, 4), "unit": "ms", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "higher_is_better": False, "dtype": "u32/f64",
+            "data": "synthetic (200 icospheres + ground, 16 of 32 materials emissive)",
+            "config": {"workload": "emitter table (compaction + prefix sums) over 1,024,002 triangles", "triangles": n,
+                       "emitters": count, "algorithmic_bytes": nb, "lights_cdf_ms": round(t_cdf * 1e3, 4),
+                       "timing": "host wall time of the synchronous call (3 launches + one count read-back)"}}
+
+
 WORKLOADS = {"trace1m": workload_trace1m, "soup10m": workload_soup10m,
+             "compat_cornell": workload_compat_cornell, "compat_spheres": workload_compat_spheres,
+             "emitters": workload_emitters,
              "render_cornell": workload_render_cornell, "render_spheres": workload_render_spheres,
              "adjoint_cornell": workload_adjoint_cornell, "batch256": workload_batch256}
 
@@ -631,6 +828,9 @@ def main() -> None:
     ap.add_argument("--workload", default="trace1m", choices=sorted(WORKLOADS))
     ap.add_argument("--tune", action="store_true", help="autotune trace launch params first")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-full", action="store_true", help="soup10m: one-thread CPU build of all 10M triangles")
+    ap.add_argument("--only", choices=["build", "incoherent", "coherent"], default=None,
+                    help="soup10m: run only the builds or one ray set (profiling)")
     ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
     ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
     ap.add_argument("--batch-only", action="store_true", help="batch256: skip the per-job side")

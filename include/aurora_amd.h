@@ -21,6 +21,7 @@
 #ifndef AURORA_AMD_H
 #define AURORA_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -169,12 +170,12 @@ aur_status aur_textures_upload(aur_context* ctx, uint32_t count, const aur_textu
                                const uint8_t* texels, uint64_t texel_bytes);
 
 /* ---- NMH build (replaces cudaRebuildNMH(Geometry&), src/kernels/rebuilder.cu:311-450,
- * declared at src/kernels/kernels.h:313).  Reorders the uploaded geometry into
+ * declared at src/kernels/kernels.h:45).  Reorders the uploaded geometry into
  * tree order on the device and prepares the traversal records. ---- */
 aur_status aur_rebuild_nmh(aur_context* ctx, aur_build_stats* stats);
 
 /* ---- tangent frames (replaces cudaGenerateTB, src/kernels/utility.cu:169-180,
- * kernels.h:303).  Runs on the tree-order geometry. ---- */
+ * src/kernels/kernels.h:35).  Runs on the tree-order geometry. ---- */
 aur_status aur_generate_tb(aur_context* ctx);
 
 /* Read back (host) for parity checks.  perm[i] = original index of the
@@ -221,7 +222,7 @@ aur_status aur_check_division(aur_context* ctx, uint64_t samples, uint64_t seed,
 
 /* ---- frame (replaces MonteCarloRaytracer::createFrame/destroyFrame/
  * setRegion/render/framebuffer, src/render/monteCarloRaytracer.cpp:23-127,
- * with the ray generation / trace / draw kernels of kernels.h:296-321). ---- */
+ * with the ray generation / trace / draw kernels of src/kernels/kernels.h:28-30,39-42,51-53). ---- */
 aur_status aur_frame_create(aur_context* ctx, uint32_t width, uint32_t height, uint32_t spp);
 aur_status aur_frame_destroy(aur_context* ctx);
 /* Clears the accumulation buffer and renders opts->spp samples. */
@@ -366,6 +367,49 @@ typedef struct aur_job {
  * Synchronous. */
 aur_status aur_batch_render(aur_context* ctx, uint32_t njobs, const aur_job* jobs, float* rgba,
                             float* job_ms);
+
+/* ---- light sampling tables (cudaCreateLightsCDF, src/kernels/light.cu:34-60;
+ * cudaCreateEmitters, src/kernels/emitter.cu:79-119; declared at
+ * src/kernels/kernels.h:56-57).  Both run on the built scene (tree order)
+ * with the current shaders and lights, write into caller-provided device
+ * memory, and are synchronous. ---- */
+
+/* Scene bounds: per-axis minimum and maximum over every vertex of the built
+ * scene (computeBounds, src/kernels/rebuilder.cu:291-309, which reads only
+ * the first two triangles in tree order; here all of them).  Host arrays of
+ * 3 floats. */
+aur_status aur_scene_bounds(aur_context* ctx, float bmin[3], float bmax[3]);
+
+/* Light-power CDF: cdf[i] = (p_0 + ... + p_i) / (p_0 + ... + p_{n-1}) over the
+ * n uploaded lights, with Light::power (src/kernels/lib/light.cuh:91-106):
+ *   ambient      4 pi lum(color) intensity (1 - ambient)
+ *   point        4 pi lum(color) intensity
+ *   directional  lum(color) intensity pi R^2, R = |bmax - bmin| / 2
+ *   area         lum(color) intensity area pi
+ * lum = 0.2126 r + 0.7152 g + 0.0722 b; each power in float as the reference
+ * writes it, the prefix sums in double, each cdf[i] rounded once to float
+ * (so cdf[n-1] = 1 exactly).  A set whose powers sum to 0 gets the uniform
+ * CDF (i + 1) / n (the reference divides by 0).  cdf: device, n floats.
+ * *count = n. */
+aur_status aur_create_lights_cdf(aur_context* ctx, float* cdf, uint32_t* count);
+
+/* Emitter record, field for field the reference's Emitter (28 B,
+ * src/util/emitter.h:10-22). */
+typedef struct aur_emitter {
+  float area;       /* triangle area, Primitive3::area (primitive.cuh:74-85) */
+  float pdf;        /* 1 / area */
+  float cdf;        /* normalised prefix sum of the emitters' shader emission */
+  float power[3];   /* the shader's emission colour */
+  uint32_t triangle; /* tree-order triangle index */
+} aur_emitter;
+/* Emissive triangles: every triangle (tree order) whose shader's scalar
+ * emission is > 0, in triangle order, with cdf the prefix sum of those
+ * emissions (not area-weighted, as the reference) divided by their total:
+ * prefix sums in double, each cdf rounded once to float.  emitters: device,
+ * capacity records; *count = the number of emitters (0: none, nothing
+ * written).  A count above capacity fails with AUR_ERR_INVALID_ARG before
+ * anything is written.  Shader ids past the shader count use shader 0. */
+aur_status aur_create_emitters(aur_context* ctx, aur_emitter* emitters, uint64_t capacity, uint32_t* count);
 
 #ifdef __cplusplus
 }

@@ -120,7 +120,9 @@ int orc_nmh_build_mt(uint32_t count, const float* vertices, uint32_t* indices_ou
     sort_triangles(count, keysMin, indices, nodes, perm, tmp, threads);
     if (i == numLevels - 1) break;
 
-    // swapMaxTriangleTPS (first maximum by position from slot +1)
+    // swapMaxTriangleTPS (first maximum by position from slot +1); splits
+    // are disjoint ranges, so they run in parallel with the same result
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64) if (threads > 1)
     for (uint32_t j = 0; j < pendingSplits; ++j) {
       const Split sp = inSplits[j];
       float maxValue = keysMax[indices[sp.index + 1]];
@@ -138,6 +140,7 @@ int orc_nmh_build_mt(uint32_t count, const float* vertices, uint32_t* indices_ou
       }
     }
     // updateNodesTPS
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64) if (threads > 1)
     for (uint32_t j = 0; j < pendingSplits; ++j) {
       const Split sp = inSplits[j];
       const uint32_t nodeIndex = doneNodes + j;

@@ -22,12 +22,32 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib():
+def use_native() -> str:
+    """Builds the oracle with -march=native for this machine
+    (oracle/_native/liboracle.so) and binds it in place of the portable
+    build; the CPU baseline times this one.  Returns the library path."""
+    global _lib
+    subprocess.run(["make", "-s", "-B", "-C", ORACLE_DIR, "native"], check=True)
+    _lib = None
+    return lib(os.path.join(ORACLE_DIR, "_native", "liboracle.so"))._name
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def lib(path: str | None = None):
     global _lib
     if _lib is None:
-        if not os.path.exists(ORACLE_SO):
+        if path is None and not os.path.exists(ORACLE_SO):
             build()
-        L = ctypes.CDLL(ORACLE_SO)
+        L = ctypes.CDLL(path or ORACLE_SO)
         P = ctypes.c_void_p
         u32, i32, u64 = ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
         L.orc_nmh_build.argtypes = [u32, P, P]
@@ -64,6 +84,12 @@ def lib():
         L.orc_sobol_owen.argtypes = [u32, u32, u32, u32, P]
         L.orc_sobol_dim1.argtypes = [u32]
         L.orc_sobol_dim1.restype = u32
+        L.orc_scene_bounds.argtypes = [u32, P, P, P]
+        L.orc_light_powers.argtypes = [u32, P, P, P, P]
+        L.orc_lights_cdf.argtypes = [u32, P, P, P, P]
+        L.orc_lights_cdf.restype = u32
+        L.orc_emitters.argtypes = [u32, P, P, u32, P, P]
+        L.orc_emitters.restype = u32
         L.orc_nested_uniform_scramble.argtypes = [u32, u32]
         L.orc_nested_uniform_scramble.restype = u32
         _lib = L
@@ -306,3 +332,40 @@ def sobol_dim1(i: int) -> int:
 
 def nested_uniform_scramble(x: int, seed: int) -> int:
     return int(lib().orc_nested_uniform_scramble(x, seed))
+
+
+EMITTER_DTYPE = np.dtype([("area", "<f4"), ("pdf", "<f4"), ("cdf", "<f4"), ("power", "<f4", 3),
+                          ("triangle", "<u4")])
+
+
+def scene_bounds(vertices: np.ndarray):
+    v = np.ascontiguousarray(vertices, np.float32)
+    lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_scene_bounds(v.shape[0], _p(v), _p(lo), _p(hi))
+    return lo, hi
+
+
+def light_powers(lights: np.ndarray, bmin, bmax) -> np.ndarray:
+    li = np.ascontiguousarray(lights)
+    lo, hi = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+    out = np.zeros(li.shape[0], np.float32)
+    lib().orc_light_powers(li.shape[0], li.ctypes.data_as(ctypes.c_void_p), _p(lo), _p(hi), _p(out))
+    return out
+
+
+def lights_cdf(lights: np.ndarray, bmin, bmax) -> np.ndarray:
+    li = np.ascontiguousarray(lights)
+    lo, hi = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+    out = np.zeros(li.shape[0], np.float32)
+    lib().orc_lights_cdf(li.shape[0], li.ctypes.data_as(ctypes.c_void_p), _p(lo), _p(hi), _p(out))
+    return out
+
+
+def emitters(tree_vertices: np.ndarray, tree_shader_ids: np.ndarray, shaders: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(tree_vertices, np.float32)
+    ids = np.ascontiguousarray(tree_shader_ids, np.uint16)
+    sh = np.ascontiguousarray(shaders)
+    out = np.zeros(v.shape[0], EMITTER_DTYPE)
+    n = lib().orc_emitters(v.shape[0], _p(v), _p(ids), sh.shape[0], sh.ctypes.data_as(ctypes.c_void_p),
+                           out.ctypes.data_as(ctypes.c_void_p))
+    return out[:n]
