@@ -67,3 +67,27 @@ def test_product_path_has_no_oracle_dependency():
     for mod in ("native.py", "renderer.py", "scenes.py"):
         text = open(os.path.join(os.path.dirname(native.LIB_PATH), mod)).read()
         assert "oracle" not in text, mod
+
+
+ABI_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "abi")
+
+
+def _gpu_present() -> bool:
+    return os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK)
+
+
+def test_header_compiles_as_c_and_cpp_and_links():
+    """tests/abi: the header as C99 (-pedantic -Werror) and C++17, record
+    sizes and offsets against the reference's layouts (static asserts),
+    every declared entry point linked from C, and the C++ Renderer class;
+    aur_create reports AUR_ERR_DEVICE where no GPU is present."""
+    subprocess.run(["make", "-s", "-B", "-C", ABI_DIR], check=True, capture_output=True)
+    out = subprocess.run([os.path.join(ABI_DIR, "c_consumer")], capture_output=True, text=True,
+                         check=True).stdout.split()
+    assert out[:2] == ["c-consumer", "ok"]
+    assert int(out[2]) == len(native.header_symbols())
+    assert int(out[4]) == native.ABI_VERSION
+    if not _gpu_present():
+        assert int(out[3]) == 3
+        res = subprocess.run([os.path.join(ABI_DIR, "cpp_consumer")], capture_output=True, text=True)
+        assert res.returncode == 0 and res.stdout.split() == ["cpp-consumer", "no-device", "3"]
