@@ -60,6 +60,9 @@ aur_status aur_create(int32_t device, aur_context** out) {
     delete ctx;
     return AUR_ERR_DEVICE;
   }
+  // device address of the allocation-free entry points' scratch on this
+  // device (once per device); on failure only those entry points refuse
+  (void)aur::scratch_setup(device);
   *out = ctx;
   return AUR_OK;
 }

@@ -268,6 +268,12 @@ struct PassThrottle {
 
 // Entry points implemented across translation units.
 namespace aur {
+// Device address of the allocation-free entry points' scratch array
+// (emitters.hip): taken once per device by scratch_setup, with that device
+// current; scratch_ptr returns it (null before setup or on failure).
+hipError_t scratch_setup(int device);
+void* scratch_ptr(int device);
+
 aur_status nmh_build(aur_context* ctx, aur_build_stats* stats);
 aur_status generate_tb(aur_context* ctx);
 aur_status trace_closest(aur_context* ctx, uint64_t n, const aur_ray* rays, aur_hit* hits);

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "context.h"
@@ -323,6 +324,37 @@ static aur_status emitters_run(aur_context* ctx, aur_emitter* out, uint64_t capa
   return AUR_OK;
 }
 
+// Tile scratch of aur_create_emitters_noalloc: a fixed device array, one
+// copy per GPU (2 MiB), for scenes of at most kNoAllocTiles tiles (2^27
+// triangles).  Host code never passes `scratch` itself to a kernel: its
+// device address on each GPU comes from hipGetSymbolAddress, taken once per
+// device at context creation (scratch_setup) with that device current.
+constexpr uint32_t kNoAllocTiles = 1u << 16;
+static __device__ __align__(256) TileSum scratch[2 * kNoAllocTiles + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static bool g_scratch_done[kMaxDevices] = {};
+
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_done[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  g_scratch_done[device] = true;
+  return hipSuccess;
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[device];
+}
+
 static aur_status tables_check(aur_context* ctx) {
   if (!ctx->built) return ctx->fail(AUR_ERR_STATE, "light tables: geometry not built");
   return AUR_OK;
@@ -384,3 +416,20 @@ aur_status aur_create_emitters(aur_context* ctx, aur_emitter* emitters, uint64_t
 }
 
 }  // extern "C"
+
+extern "C" aur_status aur_create_emitters_noalloc(aur_context* ctx, aur_emitter* emitters, uint64_t capacity,
+                                                  uint32_t* count) {
+  if (!ctx) return AUR_ERR_INVALID_ARG;
+  if (!count) return ctx->fail(AUR_ERR_INVALID_ARG, "emitters: null count");
+  aur_status st = aur::tables_check(ctx);
+  if (st != AUR_OK) return st;
+  if (ctx->num_shaders == 0) return ctx->fail(AUR_ERR_STATE, "emitters: no shaders uploaded");
+  if (capacity && !emitters) return ctx->fail(AUR_ERR_INVALID_ARG, "emitters: null output");
+  const uint32_t ntiles = (ctx->count + aur::kTile - 1) / aur::kTile;
+  if (ntiles > aur::kNoAllocTiles)
+    return ctx->fail(AUR_ERR_INVALID_ARG, "emitters (no allocation): more than 2^27 triangles");
+  AUR_HIP(ctx, hipSetDevice(ctx->device));
+  void* mem = aur::scratch_ptr(ctx->device);
+  if (!mem) return ctx->fail(AUR_ERR_DEVICE, "emitters (no allocation): device scratch address unavailable");
+  return aur::emitters_run(ctx, emitters, capacity, count, mem);
+}

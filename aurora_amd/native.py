@@ -157,6 +157,7 @@ def load():
         "aur_scene_bounds": ([P, P, P], i32),
         "aur_create_lights_cdf": ([P, P, ctypes.POINTER(u32)], i32),
         "aur_create_emitters": ([P, P, u64, ctypes.POINTER(u32)], i32),
+        "aur_create_emitters_noalloc": ([P, P, u64, ctypes.POINTER(u32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -354,6 +355,14 @@ class Context:
         n = ctypes.c_uint32()
         self._check(self._lib.aur_create_emitters(self._h, ctypes.c_void_p(out_ptr), capacity, ctypes.byref(n)),
                     "create_emitters")
+        return int(n.value)
+
+    def emitters_noalloc_ptr(self, out_ptr: int, capacity: int) -> int:
+        """aur_create_emitters_noalloc: the same table with the library's
+        fixed device scratch (no device allocation); returns the count."""
+        n = ctypes.c_uint32()
+        self._check(self._lib.aur_create_emitters_noalloc(self._h, ctypes.c_void_p(out_ptr), capacity,
+                                                          ctypes.byref(n)), "create_emitters_noalloc")
         return int(n.value)
 
     def lights_cdf(self) -> np.ndarray:

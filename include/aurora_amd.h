@@ -410,6 +410,16 @@ typedef struct aur_emitter {
  * written).  A count above capacity fails with AUR_ERR_INVALID_ARG before
  * anything is written.  Shader ids past the shader count use shader 0. */
 aur_status aur_create_emitters(aur_context* ctx, aur_emitter* emitters, uint64_t capacity, uint32_t* count);
+/* aur_create_emitters without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * and harnesses that forbid hipMalloc).  Same arguments and the same table
+ * bit for bit; its per-tile scratch is a fixed device array of the library
+ * (2 MiB on each GPU it runs on, held until the process ends), which limits
+ * it to scenes of at most 2^27 triangles (AUR_ERR_INVALID_ARG beyond).  Calls
+ * on one GPU share that array: calls that overlap on one GPU (from several
+ * threads or contexts) race. */
+aur_status aur_create_emitters_noalloc(aur_context* ctx, aur_emitter* emitters, uint64_t capacity,
+                                       uint32_t* count);
 
 #ifdef __cplusplus
 }
