@@ -537,6 +537,29 @@ def spheres(seed: int = 2, n_spheres: int = 200, subdivisions: int = 4, box: flo
                    max_depth=max_depth, rr_start=3, name=f"spheres-s{seed}-k{n_spheres}")
 
 
+def c4_jobs(n_jobs: int = 256, seed: int = 44, sides=(64, 128, 256, 384, 512), spps=(4, 16, 64)) -> list:
+    """Preview jobs of the batched workload: Cornell scenes with 2..40
+    subdivision-4 spheres (10,264-204,824 triangles), square sides drawn from
+    `sides`, spp from `spps`, depth 6, Russian roulette from bounce 3."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_jobs):
+        side = int(rng.choice(list(sides)))
+        spp = int(rng.choice(list(spps)))
+        n_sph = int(rng.integers(2, 41))
+        out.append(cornell(seed=1000 + k, n_spheres=n_sph, subdivisions=4, width=side, height=side,
+                           spp=spp, max_depth=6))
+    return out
+
+
+def c5_scene(width: int = 512, height: int = 512) -> Scene:
+    """The differentiable-rendering scene: Cornell box with 20 subdivision-4
+    spheres (102,424 triangles), every wall triangle and sphere half its own
+    Lambert shader (64), 4 ceiling lights, depth 6."""
+    return cornell(n_spheres=20, subdivisions=4, width=width, height=height, spp=64, max_depth=6,
+                   n_lights=4, split_materials=True)
+
+
 def soup(seed: int = 3, n: int = 10_000_000, box: float = 1000.0, chunk: int = 1 << 20) -> Scene:
     """Random equilateral-triangle soup (centroids uniform in the box,
     edge ~ LogNormal(0, 1) m, uniformly random orientation)."""

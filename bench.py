@@ -436,10 +436,7 @@ def workload_render_spheres(a) -> dict:
 def c5_scene():
     from aurora_amd import scenes
 
-    # Cornell box with 20 subdivision-4 spheres (102,424 triangles), every wall
-    # triangle and sphere half its own Lambert shader (64), 4 ceiling lights
-    return scenes.cornell(n_spheres=20, subdivisions=4, width=512, height=512, spp=64, max_depth=6,
-                          n_lights=4, split_materials=True)
+    return scenes.c5_scene()
 
 
 def workload_adjoint_cornell(a) -> dict:
@@ -580,21 +577,9 @@ def workload_adjoint_cornell(a) -> dict:
 
 
 def c4_jobs(n_jobs: int = 256, seed: int = 44):
-    """256 preview jobs: Cornell scenes with 2..40 subdivision-4 spheres
-    (10k-205k triangles), square sides in {64,128,256,384,512}, spp in
-    {4,16,64}, depth 6, Russian roulette from bounce 3."""
     from aurora_amd import scenes
 
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(n_jobs):
-        side = int(rng.choice([64, 128, 256, 384, 512]))
-        spp = int(rng.choice([4, 16, 64]))
-        n_sph = int(rng.integers(2, 41))
-        s = scenes.cornell(seed=1000 + k, n_spheres=n_sph, subdivisions=4, width=side, height=side,
-                           spp=spp, max_depth=6)
-        out.append(s)
-    return out
+    return scenes.c4_jobs(n_jobs, seed)
 
 
 def workload_batch256(a) -> dict:
@@ -722,6 +707,8 @@ def compat_frame(a, scene, label: str) -> dict:
     ctx = native.Context(0)
     ctx.upload_scene(scene)
     ctx.rebuild_nmh()
+    if a.refill:
+        ctx.set_render_params(refill=a.refill)
     w, h, spp = scene.width, scene.height, a.spp or scene.spp
     ctx.frame_create(w, h, spp)
     o = lambda seed: native.Context.opts(native.MODE_COMPAT, spp, scene.max_depth, scene.rr_start, seed)  # noqa: E731
@@ -738,11 +725,12 @@ def compat_frame(a, scene, label: str) -> dict:
     ctx.frame_render(scene.camera, o(8))
     img_b = ctx.frame_read()
     err = two_seed_rel_rmse(img_a, img_b)
-    ctx.close()
     t = float(np.median(times))
+    params = ctx.get_render_params()
+    ctx.close()
     log(f"{label}: compat {w}x{h} {spp} spp: {t * 1e3:.1f} ms, {rays / t / 1e6:.0f} Mrays/s, rel-RMSE {err:.4f}")
     return {"ms": round(t * 1e3, 3), "mrays": round(rays / t / 1e6, 1), "rays": int(rays), "spp": spp,
-            "rel_rmse_two_seed": round(err, 5), "runs": len(times)}
+            "rel_rmse_two_seed": round(err, 5), "runs": len(times), "render_params": params}
 
 
 def workload_compat_cornell(a) -> dict:
@@ -833,6 +821,7 @@ def main() -> None:
                     help="soup10m: run only the builds or one ray set (profiling)")
     ap.add_argument("--max-paths", type=int, default=0, help="render: paths in flight per pass")
     ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
+    ap.add_argument("--refill", type=int, default=0, help="render: refill threshold of the persistent launches")
     ap.add_argument("--batch-only", action="store_true", help="batch256: skip the per-job side")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
