@@ -49,10 +49,6 @@ struct ChildDesc {
 struct GroupDesc {
   uint32_t off, size, tb, te, cb, ce;  // tiles [tb, te), offset chunks [cb, ce)
 };
-// Up to kChunk consecutive tiles of one group: the unit of the offset scan.
-struct ChunkDesc {
-  uint32_t tb, te;
-};
 struct SubGroup {
   uint32_t off, size, first, count;
 };
@@ -60,9 +56,11 @@ struct SubGroup {
 // Triangle bounds on one axis, (min, max), stored [axis][triangle].
 // kx, ky: the min keys of x and y on the axis of the last level (L - 1),
 // loaded by the caller; level <= L - 2.
+// orig (optional): triangle id of each renumbered index x, y (the subtree
+// phase works on positions, renumbered by the last k_emit).
 __device__ inline void write_pair_k(uint32_t h, uint32_t x, uint32_t y, float kx, float ky,
                                     int level, int L, const float2* __restrict__ bb, uint32_t N,
-                                    uint32_t* __restrict__ P) {
+                                    uint32_t* __restrict__ P, const uint32_t* __restrict__ orig = nullptr) {
   bool y_first = ky < kx;
   if (!y_first && !(kx < ky)) {
     const int lo = max(level + 1, L - 3);
@@ -76,8 +74,9 @@ __device__ inline void write_pair_k(uint32_t h, uint32_t x, uint32_t y, float kx
       if (ax < ay) break;
     }
   }
-  P[2 * (size_t)h] = y_first ? y : x;
-  P[2 * (size_t)h + 1] = y_first ? x : y;
+  const uint32_t a = y_first ? y : x, b = y_first ? x : y;
+  P[2 * (size_t)h] = orig ? orig[a] : a;
+  P[2 * (size_t)h + 1] = orig ? orig[b] : b;
 }
 
 __device__ inline void write_pair(uint32_t h, uint32_t x, uint32_t y, int level, int L,
@@ -120,10 +119,16 @@ __global__ void k_fill0(uint32_t* __restrict__ p, uint32_t N) {
   if (i < N) p[i] = 0;
 }
 
+// Per-tile digit histogram H[tile][digit], also added into its chunk's digit
+// counts CS[chunk][digit] (integer atomics: the sums are exact whatever the
+// order).  CS starts at zero: zeroed once per build and again by
+// k_radix_group_scan after it reads a pass's counts.
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restrict__ tiles,
+                                                         const GroupDesc* __restrict__ groups,
                                                          const uint32_t* __restrict__ K,
                                                          uint32_t shift,
-                                                         uint32_t* __restrict__ H) {
+                                                         uint32_t* __restrict__ H,
+                                                         uint32_t* __restrict__ CS) {
   __shared__ uint32_t h[256];
   const TileDesc t = tiles[blockIdx.x];
   h[threadIdx.x] = 0;
@@ -131,26 +136,18 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restr
   for (uint32_t i = threadIdx.x; i < t.count; i += kThreads)
     atomicAdd(&h[(K[t.start + i] >> shift) & 255u], 1u);
   __syncthreads();
-  H[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+  const uint32_t v = h[threadIdx.x];
+  H[(size_t)blockIdx.x * 256 + threadIdx.x] = v;
+  const GroupDesc g = groups[t.group];
+  const uint32_t chunk = g.cb + (blockIdx.x - g.tb) / kChunk;
+  if (v) atomicAdd(&CS[(size_t)chunk * 256 + threadIdx.x], v);
 }
 
-// Digit-major, tile-minor exclusive offsets of every tile (stable scatter),
-// as a reduce-then-scan over chunks of tiles so that no thread walks a whole
-// group: chunk sums, one scan per group over its chunks, then the tiles of
-// each chunk.
-__global__ __launch_bounds__(kThreads) void k_radix_chunk_sums(const ChunkDesc* __restrict__ chunks,
-                                                               const uint32_t* __restrict__ H,
-                                                               uint32_t* __restrict__ S) {
-  const ChunkDesc c = chunks[blockIdx.x];
-  const uint32_t d = threadIdx.x;
-  uint32_t sum = 0;
-#pragma unroll 8
-  for (uint32_t t = c.tb; t < c.te; ++t) sum += H[(size_t)t * 256 + d];
-  S[(size_t)blockIdx.x * 256 + d] = sum;
-}
-
+// Per group: exclusive digit offsets of the group (digit-major), and the base
+// B[chunk][digit] of every chunk of the group (the counts of the group's
+// earlier chunks added); then the chunk counts are zeroed for the next pass.
 __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* __restrict__ groups,
-                                                               const uint32_t* __restrict__ S,
+                                                               uint32_t* __restrict__ S,
                                                                uint32_t* __restrict__ B) {
   __shared__ uint32_t s[256];
   const GroupDesc g = groups[blockIdx.x];
@@ -171,20 +168,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
   for (uint32_t c = g.cb; c < g.ce; ++c) {
     B[(size_t)c * 256 + d] = run;
     run += S[(size_t)c * 256 + d];
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const ChunkDesc* __restrict__ chunks,
-                                                                  const uint32_t* __restrict__ H,
-                                                                  const uint32_t* __restrict__ B,
-                                                                  uint32_t* __restrict__ O) {
-  const ChunkDesc c = chunks[blockIdx.x];
-  const uint32_t d = threadIdx.x;
-  uint32_t run = B[(size_t)blockIdx.x * 256 + d];
-#pragma unroll 8
-  for (uint32_t t = c.tb; t < c.te; ++t) {
-    O[(size_t)t * 256 + d] = run;
-    run += H[(size_t)t * 256 + d];
+    S[(size_t)c * 256 + d] = 0u;
   }
 }
 
@@ -197,6 +181,23 @@ __global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const ChunkDes
 // run is one contiguous store.  (Staging only a 16-bit tile index and
 // re-reading the payload at write-out measured 6% slower despite the
 // smaller LDS footprint.)
+// Output offsets of every tile, O[tile][digit]: the chunk's base plus the
+// counts of the chunk's earlier tiles (one block per chunk; the chunk's
+// tiles are found from the tile's group).
+__global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const uint32_t* __restrict__ chunk_tile,
+                                                                  const uint32_t* __restrict__ H,
+                                                                  const uint32_t* __restrict__ B,
+                                                                  uint32_t* __restrict__ O) {
+  const uint32_t tb = chunk_tile[blockIdx.x], te = chunk_tile[blockIdx.x + 1];
+  const uint32_t d = threadIdx.x;
+  uint32_t run = B[(size_t)blockIdx.x * 256 + d];
+#pragma unroll 8
+  for (uint32_t t = tb; t < te; ++t) {
+    O[(size_t)t * 256 + d] = run;
+    run += H[(size_t)t * 256 + d];
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
     const uint2* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ H,
@@ -339,7 +340,11 @@ __global__ __launch_bounds__(kThreads) void k_argmax_combine(const ChildDesc* __
 
 // Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders;
 // with Knext, also the next level's sort keys and max keys of the compacted
-// remainders (one 8-byte load of their bounds on the next axis).
+// remainders (one 8-byte load of their bounds on the next axis).  With orig
+// (the last level before the subtree phase), the remainders are renumbered
+// instead: position dst takes the place of triangle id val, orig[dst] = val,
+// and its bounds on all three axes go to bbR[axis][dst] (mR per axis), so
+// that every subtree group reads one contiguous window of bounds.
 __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ tiles,
                                                    const ChildDesc* __restrict__ children,
                                                    const uint32_t* __restrict__ argpos,
@@ -347,7 +352,9 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
                                                    uint2* __restrict__ Vnext, uint32_t first,
                                                    int level, int L, const float2* __restrict__ bb,
                                                    uint32_t N, uint32_t* __restrict__ P,
-                                                   uint32_t* __restrict__ Knext) {
+                                                   uint32_t* __restrict__ Knext,
+                                                   float2* __restrict__ bbR, uint32_t* __restrict__ orig,
+                                                   uint32_t mR) {
   const TileDesc t = tiles[blockIdx.x];
   const ChildDesc c = children[t.child];
   const float2* bnext = bb + (size_t)((level + 1) % 3) * N;
@@ -364,6 +371,15 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
         const float2 b = bnext[val];
         Knext[dst] = fkey(b.x);
         Vnext[dst] = make_uint2(val, fkey(b.y));
+      } else if (orig) {
+        const float2 b0 = bb[val], b1 = bb[(size_t)N + val], b2 = bb[2 * (size_t)N + val];
+        if (dst < mR) {
+          orig[dst] = val;
+          bbR[dst] = b0;
+          bbR[(size_t)mR + dst] = b1;
+          bbR[2 * (size_t)mR + dst] = b2;
+        }
+        Vnext[dst] = make_uint2(dst, 0u);
       } else {
         Vnext[dst] = make_uint2(val, 0u);
       }
@@ -372,12 +388,16 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
 }
 
 // One workgroup per group: all remaining levels of the group's subtree(s).
+// With orig, the pending region is renumbered (by the last k_emit): element values
+// are positions, bb is bbR with stride N = m, and orig maps positions back to
+// triangle ids where the tree order is written.
 // 37 KB of LDS and at most 128 VGPRs, so four groups reside per CU.
 __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restrict__ groups,
                                                          const uint2* __restrict__ V,
                                                          const float2* __restrict__ bb, uint32_t N,
                                                          uint32_t n, int L, int level0,
-                                                         uint32_t* __restrict__ P) {
+                                                         uint32_t* __restrict__ P,
+                                                         const uint32_t* __restrict__ orig) {
   __shared__ uint32_t E[kSubT];
   __shared__ uint16_t G[kSubT];
   __shared__ unsigned long long K[kSubT];
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   uint32_t m = g.size;
   for (uint32_t i = tid; i < m; i += kThreads) {
-    E[i] = clampi(V[g.off + i].x, N);
+    E[i] = orig ? g.off + i : clampi(V[g.off + i].x, N);
     G[i] = 0;
   }
   uint32_t f = g.first, c = g.count;
@@ -499,8 +519,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
     if (last) {
       for (uint32_t j = tid; j < c; j += kThreads) {
         const uint32_t o = nodeOff[j];
-        P[2 * (size_t)(f + j)] = E2[o];
-        P[2 * (size_t)(f + j) + 1] = E2[o + 1];
+        P[2 * (size_t)(f + j)] = orig ? orig[E2[o]] : E2[o];
+        P[2 * (size_t)(f + j) + 1] = orig ? orig[E2[o + 1]] : E2[o + 1];
       }
       break;
     }
@@ -541,7 +561,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
 #pragma unroll
       for (int r = 0; r < kPairs; ++r) {
         const uint32_t j = tid + r * kThreads;
-        if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bb, N, P);
+        if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bb, N, P, orig);
       }
     }
     for (uint32_t i = tid; i < m; i += kThreads) {
@@ -723,7 +743,8 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   std::vector<TileDesc> tiles;
   std::vector<ChildDesc> children;
   std::vector<GroupDesc> groups;
-  std::vector<ChunkDesc> chunks;
+  uint32_t chunk_count = 0;  // chunks of kChunk tiles of one group, per level
+  std::vector<uint32_t> chunk_tiles;  // per level: first tile of each chunk, then the level's tile count
   std::vector<LevelPlan> plan;
   size_t max_tiles = 1, max_children = 1, max_chunks = 1;
   for (int l = 0; l < level0; ++l) {
@@ -760,14 +781,17 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     lp.nchildren = (uint32_t)(children.size() - lp.child_base);
     lp.ngroups = (uint32_t)(groups.size() - lp.group_base);
     lp.ntiles = (uint32_t)(tiles.size() - lp.tile_base);
-    lp.chunk_base = chunks.size();
+    chunk_count = 0;
+    lp.chunk_base = chunk_tiles.size();
     for (size_t gi = lp.group_base; gi < groups.size(); ++gi) {
       GroupDesc& g = groups[gi];
-      g.cb = (uint32_t)(chunks.size() - lp.chunk_base);
-      for (uint32_t t = g.tb; t < g.te; t += kChunk) chunks.push_back(ChunkDesc{t, std::min(t + kChunk, g.te)});
-      g.ce = (uint32_t)(chunks.size() - lp.chunk_base);
+      g.cb = chunk_count;
+      for (uint32_t t = g.tb; t < g.te; t += kChunk) chunk_tiles.push_back(t);
+      chunk_count += (g.te - g.tb + kChunk - 1) / kChunk;
+      g.ce = chunk_count;
     }
-    lp.nchunks = (uint32_t)(chunks.size() - lp.chunk_base);
+    chunk_tiles.push_back(lp.ntiles);
+    lp.nchunks = chunk_count;
     max_chunks = std::max<size_t>(max_chunks, lp.nchunks);
     max_tiles = std::max<size_t>(max_tiles, lp.ntiles);
     max_children = std::max<size_t>(max_children, lp.nchildren);
@@ -792,13 +816,13 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
 
   // ---- device buffers ----
   DevBuf<float2> bb;          // (min, max) per axis and triangle
+  DevBuf<float2> bbR;         // the same for the subtree phase's pending region, by position
   DevBuf<uint2> Va, Vb;        // sort payload: (triangle id, max key on the level's axis)
-  DevBuf<uint32_t> Ka, Kb, H, O, CS, CB, argpos;
+  DevBuf<uint32_t> Ka, Kb, H, O, CS, CB, argpos, dchunk_tiles;
   DevBuf<uint2> partial;
   DevBuf<TileDesc> dtiles;
   DevBuf<ChildDesc> dchildren;
   DevBuf<GroupDesc> dgroups;
-  DevBuf<ChunkDesc> dchunks;
   DevBuf<SubGroup> dsub;
   AUR_HIP(ctx, bb.reserve((size_t)3 * N));
   AUR_HIP(ctx, Va.reserve(N));
@@ -809,10 +833,11 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     AUR_HIP(ctx, Kb.reserve(N));
     AUR_HIP(ctx, H.reserve(max_tiles * 256));
     AUR_HIP(ctx, O.reserve(max_tiles * 256));
+    AUR_HIP(ctx, dchunk_tiles.reserve(chunk_tiles.size()));
+    AUR_HIP(ctx, hipMemcpy(dchunk_tiles.p, chunk_tiles.data(), chunk_tiles.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice));
     AUR_HIP(ctx, CS.reserve(max_chunks * 256));
     AUR_HIP(ctx, CB.reserve(max_chunks * 256));
-    AUR_HIP(ctx, dchunks.reserve(chunks.size()));
-    AUR_HIP(ctx, hipMemcpy(dchunks.p, chunks.data(), chunks.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice));
     AUR_HIP(ctx, partial.reserve(max_tiles));
     AUR_HIP(ctx, argpos.reserve(max_children));
     AUR_HIP(ctx, dtiles.reserve(tiles.size()));
@@ -824,6 +849,9 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     AUR_HIP(ctx, hipMemcpy(dgroups.p, groups.data(), groups.size() * sizeof(GroupDesc),
                            hipMemcpyHostToDevice));
   }
+  // pending triangles when the subtree phase starts
+  const uint32_t m0 = level0 > 0 ? (uint32_t)(N - 2 * ((1ull << level0) - 1)) : N;
+  if (level0 > 0) AUR_HIP(ctx, bbR.reserve((size_t)3 * m0));
   AUR_HIP(ctx, dsub.reserve(sub.size()));
   AUR_HIP(ctx, hipMemcpy(dsub.p, sub.data(), sub.size() * sizeof(SubGroup), hipMemcpyHostToDevice));
 
@@ -837,6 +865,7 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   // and max keys are gathered by the previous level's emit pass
   k_iota<<<blocks(N, 256), 256, 0, s>>>(Va.p, N, bb.p, plan.empty() ? nullptr : Ka.p);
   k_fill0<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N);
+  if (!plan.empty()) k_fill0<<<blocks(max_chunks * 256, 256), 256, 0, s>>>(CS.p, (uint32_t)(max_chunks * 256));
   uint2* Vcur = Va.p;
   uint2* Valt = Vb.p;
   for (size_t li = 0; li < plan.size(); ++li) {
@@ -844,15 +873,13 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     const TileDesc* tl = dtiles.p + lp.tile_base;
     const ChildDesc* ch = dchildren.p + lp.child_base;
     const GroupDesc* gr = dgroups.p + lp.group_base;
-    const ChunkDesc* ck = dchunks.p + lp.chunk_base;
     uint32_t *kin = Ka.p, *kout = Kb.p;
     uint2 *vin = Vcur, *vout = Valt;
     for (int pass = 0; pass < 4; ++pass) {
       const uint32_t shift = 8u * pass;
-      k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, shift, H.p);
-      k_radix_chunk_sums<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CS.p);
+      k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, gr, kin, shift, H.p, CS.p);
       k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, CS.p, CB.p);
-      k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(ck, H.p, CB.p, O.p);
+      k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(dchunk_tiles.p + lp.chunk_base, H.p, CB.p, O.p);
       k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, H.p, O.p, lp.m, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);
@@ -861,14 +888,21 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
     k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Vcur, partial.p);
     k_argmax_combine<<<blocks(lp.nchildren, kThreads / 64), kThreads, 0, s>>>(ch, lp.nchildren,
                                                                               partial.p, argpos.p);
+    const bool last = li + 1 == plan.size();
     k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Vcur, Valt, lp.first, lp.level, L,
-                                          bb.p, N, ctx->perm.p,
-                                          li + 1 < plan.size() ? Ka.p : nullptr);
+                                          bb.p, N, ctx->perm.p, last ? nullptr : Ka.p,
+                                          last ? bbR.p : nullptr, last ? Ka.p : nullptr, m0);
     std::swap(Vcur, Valt);
   }
   AUR_LAUNCH_CHECK(ctx, "nmh level kernels");
-  k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Vcur, bb.p, N, n, L, level0,
-                                                      ctx->perm.p);
+  if (level0 > 0) {
+    // the subtree phase on positions (renumbered by the last k_emit)
+    k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Vcur, bbR.p, m0, n, L, level0,
+                                                        ctx->perm.p, Ka.p);
+  } else {
+    k_subtree<<<(uint32_t)sub.size(), kThreads, 0, s>>>(dsub.p, Vcur, bb.p, N, n, L, level0,
+                                                        ctx->perm.p, nullptr);
+  }
   AUR_LAUNCH_CHECK(ctx, "k_subtree");
 
   AUR_HIP(ctx, ctx->vertices.reserve((size_t)N * 9));
