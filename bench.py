@@ -389,9 +389,22 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
             hit = rows[-1]
             break
         spp *= 2
+    ref = None
+    if hit and not a.spp:
+        # against a converged reference: R at 16x the spp (seed 9); R's own
+        # variance (1/16 of the image's) is taken out of the distance
+        rs = 16 * hit["spp"]
+        ctx.frame_create(w, h, rs)
+        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, rs, scene.max_depth, scene.rr_start, 9))
+        R = ctx.frame_read()[:, :3].astype(np.float64)
+        d = float(np.linalg.norm(img_a[:, :3] - R) / np.linalg.norm(R))
+        ref = {"reference_spp": rs, "rel_rmse_vs_reference": round(d, 5),
+               "rel_rmse_vs_reference_corrected": round(d / np.sqrt(1.0 + 1.0 / 16.0), 5)}
+        log(f"{label}: {hit['spp']} spp vs {rs}-spp reference: rel-RMSE {d:.4f} "
+            f"({d / np.sqrt(1 + 1 / 16):.4f} without the reference's variance)")
     params = ctx.get_render_params()
     ctx.close()
-    return {"curve": rows, "to_1pct": hit, "render_params": params,
+    return {"curve": rows, "to_1pct": hit, "render_params": params, "against_reference": ref,
             "error_estimate": "two-seed: ||I7 - I8|| / (sqrt2 ||(I7 + I8)/2||)"}
 
 
@@ -656,8 +669,24 @@ def workload_batch256(a) -> dict:
         runs.append((time.perf_counter() - t1, job_ms))
     T_batch, job_ms = min(runs, key=lambda r: r[0])
     rays_batch = ctx.rays_traced()
+    # per-job error: a second batch at independent seeds gives every job's
+    # two-seed rel-RMSE at its own spp, and the spp it would need for 1%
+    # (error ~ 1 / sqrt(spp))
+    img_a = out.cpu().numpy()
+    jobs_b = jobs.copy()
+    jobs_b["seed"] = jobs["seed"] + 100_000
+    ctx.batch_render_ptr(jobs_b, out.data_ptr())
+    img_b = out.cpu().numpy()
+    job_err = []
+    for k, s in enumerate(sc):
+        o0, n = int(jobs[k]["out_offset"]), s.width * s.height
+        job_err.append(two_seed_rel_rmse(img_a[o0:o0 + n], img_b[o0:o0 + n]))
+    job_err = np.array(job_err)
+    spp_1pct = np.array([s.spp for s in sc]) * (job_err / 0.01) ** 2
+    log(f"per-job two-seed rel-RMSE at the jobs' spp: min {job_err.min():.4f} median {np.median(job_err):.4f} "
+        f"max {job_err.max():.4f}; spp for 1%: median {np.median(spp_1pct):.0f}")
     log(f"batched: {T_batch:.3f}s, {len(sc) / T_batch:.1f} jobs/s, {rays_batch / T_batch / 1e6:.0f} Mrays/s")
-    img = out.cpu().numpy()
+    img = img_a
     same = 0
     for k in check if not a.batch_only else ():
         o0 = int(jobs[k]["out_offset"])
@@ -693,6 +722,11 @@ def workload_batch256(a) -> dict:
             "per_job_mrays": round(rays_single / T_single / 1e6, 1),
             "identical_images": f"{same}/{len(check)}",
             "render_params": params,
+            "per_job_rel_rmse_two_seed": {"min": round(float(job_err.min()), 5),
+                                          "median": round(float(np.median(job_err)), 5),
+                                          "max": round(float(job_err.max()), 5),
+                                          "jobs_at_or_below_1pct": int((job_err <= 0.01).sum())},
+            "per_job_spp_for_1pct_median": round(float(np.median(spp_1pct)), 1),
         },
     }
 
