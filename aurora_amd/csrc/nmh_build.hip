@@ -172,15 +172,6 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
   }
 }
 
-// Stable scatter, staged through LDS.  Elements of a tile are ranked in order
-// within their digit: rounds of 256 elements; inside a wave, lanes with the
-// same digit are found with 8 ballots; per-wave digit counts go through LDS in
-// wave order.  Each element (key and 8-byte payload) lands in LDS at its
-// digit-sorted tile position (the tile's own exclusive digit offsets, from
-// its histogram), and the tile is then written out in that order: a digit's
-// run is one contiguous store.  (Staging only a 16-bit tile index and
-// re-reading the payload at write-out measured 6% slower despite the
-// smaller LDS footprint.)
 // Output offsets of every tile, O[tile][digit]: the chunk's base plus the
 // counts of the chunk's earlier tiles (one block per chunk; the chunk's
 // tiles are found from the tile's group).
@@ -198,42 +189,63 @@ __global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const uint32_t
   }
 }
 
+// Inclusive scan of one value per thread over the 256 threads (4 waves):
+// shuffles within each wave, then the wave totals through LDS.
+__device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* __restrict__ wsum) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off);
+    if (lane >= (uint32_t)off) v += u;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  for (uint32_t w = 0; w < wave; ++w) v += wsum[w];
+  return v;
+}
+
+// Stable scatter of one 2048-element tile, staged through LDS.  Ranking:
+// wave w owns the tile's elements [512w, 512w + 512) in rounds of 64; lanes
+// with the same digit are found with 8 ballots, and the wave keeps its
+// running count per digit in its own LDS row (read, then written by the
+// digit's first lane, in the wave's program order: no workgroup barrier per
+// round).  Each element (key and 8-byte payload) then lands in LDS at its
+// digit-sorted tile position, and the tile is written out in that order: a
+// digit's run is one contiguous store at the tile's output offset for the
+// digit, O[tile][digit].  (Ranking in rounds of 256 elements across the
+// workgroup, with four barriers per round, took 20-35% longer; staging only
+// a 16-bit tile index and re-reading the payload at write-out measured 6%
+// slower despite the smaller LDS footprint.)
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
-    const uint2* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ H,
-    const uint32_t* __restrict__ O, uint32_t m, uint32_t* __restrict__ Kout,
-    uint2* __restrict__ Vout) {
+    const uint2* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ O, uint32_t m,
+    uint32_t* __restrict__ Kout, uint2* __restrict__ Vout) {
+  constexpr uint32_t kWaves = kThreads / 64;
+  constexpr uint32_t kRounds = kTile / kThreads;  // rounds of 64 per wave
   __shared__ uint32_t sk[kTile];
   __shared__ uint2 sv[kTile];
-  __shared__ uint32_t base[256];   // next free tile position per digit
-  __shared__ uint32_t delta[256];  // output position - tile position, per digit
-  __shared__ uint32_t wcnt[kThreads / 64][256];
-  const TileDesc t = tiles[blockIdx.x];
+  __shared__ uint32_t wc[kWaves][256];  // per-wave running digit counts, then tile positions
+  __shared__ uint32_t delta[256];       // output position - tile position, per digit
+  __shared__ uint32_t wsum[kWaves];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t cnt = H[(size_t)blockIdx.x * 256 + tid];
-  // exclusive scan of the tile histogram: digit tid's first tile position
-  base[tid] = cnt;
-  __syncthreads();
-  for (uint32_t off = 1; off < 256; off <<= 1) {
-    const uint32_t v = tid >= off ? base[tid - off] : 0u;
-    __syncthreads();
-    base[tid] += v;
-    __syncthreads();
-  }
-  const uint32_t first = base[tid] - cnt;
-  __syncthreads();
-  base[tid] = first;
-  delta[tid] = O[(size_t)blockIdx.x * 256 + tid] - first;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t r0 = 0; r0 < t.count; r0 += kThreads) {
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) wcnt[w][tid] = 0;
-    __syncthreads();
-    const uint32_t i = r0 + tid;
+  for (uint32_t q = 0; q < 4; ++q) wc[wave][q * 64 + lane] = 0;
+  const TileDesc t = tiles[blockIdx.x];
+  const uint32_t w0 = wave * (kRounds * 64);
+  uint32_t k[kRounds], loc[kRounds];
+  uint2 v[kRounds];
+#pragma unroll
+  for (uint32_t r = 0; r < kRounds; ++r) {
+    const uint32_t i = w0 + r * 64 + lane;
     const bool valid = i < t.count;
-    const uint32_t k = valid ? Kin[t.start + i] : 0u;
-    const uint2 v = valid ? Vin[t.start + i] : make_uint2(0u, 0u);
-    const uint32_t dg = (k >> shift) & 255u;
+    k[r] = valid ? Kin[t.start + i] : 0u;
+    v[r] = valid ? Vin[t.start + i] : make_uint2(0u, 0u);
+  }
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (uint32_t r = 0; r < kRounds; ++r) {
+    const bool valid = w0 + r * 64 + lane < t.count;
+    const uint32_t dg = (k[r] >> shift) & 255u;
     uint64_t mask = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -242,29 +254,47 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
       mask &= bit ? bb : ~bb;
     }
     const uint32_t rank = __popcll(mask & lt_mask);
-    if (valid && rank == 0) wcnt[wave][dg] = __popcll(mask);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = base[dg] + rank;
-      for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][dg];
+    const uint32_t off = wc[wave][dg];
+    if (valid && rank == 0) wc[wave][dg] = off + (uint32_t)__popcll(mask);
+    loc[r] = off + rank;
+  }
+  __syncthreads();
+  // thread tid = digit d: the tile's count, its first tile position, and
+  // the first tile position of each wave's elements with digit d
+  uint32_t c[kWaves], cnt = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kWaves; ++w) {
+    c[w] = wc[w][tid];
+    cnt += c[w];
+  }
+  const uint32_t first = block_incl_scan256(cnt, wsum) - cnt;
+  {
+    uint32_t run = first;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      wc[w][tid] = run;
+      run += c[w];
+    }
+  }
+  delta[tid] = O[(size_t)blockIdx.x * 256 + tid] - first;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kRounds; ++r) {
+    if (w0 + r * 64 + lane < t.count) {
+      const uint32_t pos = wc[wave][(k[r] >> shift) & 255u] + loc[r];
       if (pos < kTile) {
-        sk[pos] = k;
-        sv[pos] = v;
+        sk[pos] = k[r];
+        sv[pos] = v[r];
       }
     }
-    __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) add += wcnt[w][tid];
-    base[tid] += add;
-    __syncthreads();
   }
+  __syncthreads();
   const uint32_t n_out = min(t.count, kTile);
   for (uint32_t j = tid; j < n_out; j += kThreads) {
-    const uint32_t k = sk[j];
-    const uint32_t pos = delta[(k >> shift) & 255u] + j;
+    const uint32_t key = sk[j];
+    const uint32_t pos = delta[(key >> shift) & 255u] + j;
     if (pos < m) {
-      Kout[pos] = k;
+      Kout[pos] = key;
       Vout[pos] = sv[j];
     }
   }
@@ -880,7 +910,7 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
       k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, gr, kin, shift, H.p, CS.p);
       k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, CS.p, CB.p);
       k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(dchunk_tiles.p + lp.chunk_base, H.p, CB.p, O.p);
-      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, H.p, O.p, lp.m, kout, vout);
+      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, O.p, lp.m, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);
     }

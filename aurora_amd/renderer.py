@@ -1,8 +1,8 @@
 """Host-side mirror of the reference's renderer layer over the C ABI.
 
 Same names, argument meaning and status behaviour as
-  Renderer            src/core/renderer.h:21-42
   MonteCarloRaytracer src/render/monteCarloRaytracer.h:14-40, .cpp:23-127
+                      (and the Renderer interface, src/core/renderer.h:21-42)
   Scene::update       src/core/scene.cpp:398-449 (mesh/shader/light upload,
                       rebuild, generateTB)
 with Maya's MStatus replaced by the Status values below and MDagPath by an
@@ -71,30 +71,10 @@ class Scene:
         return Status.kSuccess
 
 
-class Renderer:
-    """Abstract renderer (src/core/renderer.h:21-42)."""
-
-    def createFrame(self, width, height, samples, scene, camera) -> Status:  # noqa: N802
-        raise NotImplementedError
-
-    def destroyFrame(self) -> Status:  # noqa: N802
-        raise NotImplementedError
-
-    def setRegion(self, region: Rect) -> Status:  # noqa: N802
-        raise NotImplementedError
-
-    def update(self) -> Status:
-        return Status.kSuccess
-
-    def render(self, engine=None, ipr: bool = False) -> Status:
-        raise NotImplementedError
-
-    def framebuffer(self):
-        raise NotImplementedError
-
-
-class MonteCarloRaytracer(Renderer):
-    """src/render/monteCarloRaytracer.cpp:23-127 over aur_frame_*.
+class MonteCarloRaytracer:
+    """src/render/monteCarloRaytracer.cpp:23-127 over aur_frame_*, with the
+    Renderer interface it implements (src/core/renderer.h:21-42: createFrame,
+    destroyFrame, setRegion, update, render, framebuffer, filterPixels).
 
     mode selects the reference-compatible estimator (native.MODE_COMPAT) or
     the path loop (native.MODE_PBR); seed keys the stateless random numbers
@@ -190,6 +170,10 @@ class MonteCarloRaytracer(Renderer):
         except native.AuroraError as e:
             return _status_of(e)
         self.m_done, self.m_total = int(cp["samples_done"]), int(cp["total"])
+        return Status.kSuccess
+
+    def update(self) -> Status:
+        """Renderer::update: nothing to refresh (scene data lives in the context)."""
         return Status.kSuccess
 
     def render(self, engine=None, ipr: bool = False) -> Status:

@@ -391,12 +391,21 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
         spp *= 2
     ref = None
     if hit and not a.spp:
-        # against a converged reference: R at 16x the spp (seed 9); R's own
-        # variance (1/16 of the image's) is taken out of the distance
+        # against a converged reference: R at 16x the spp; R's own variance
+        # (1/16 of the image's) is taken out of the distance.  One render
+        # holds fewer than 2^32 pixel samples, so R is the mean of equal
+        # renders with seeds 9, 10, ...
         rs = 16 * hit["spp"]
-        ctx.frame_create(w, h, rs)
-        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, rs, scene.max_depth, scene.rr_start, 9))
-        R = ctx.frame_read()[:, :3].astype(np.float64)
+        part = rs
+        while part > 1 and w * h * part >= 1 << 32:
+            part //= 2
+        ctx.frame_create(w, h, part)
+        R = np.zeros((w * h, 3))
+        for k in range(rs // part):
+            ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, part, scene.max_depth,
+                                                               scene.rr_start, 9 + k))
+            R += ctx.frame_read()[:, :3]
+        R /= rs // part
         d = float(np.linalg.norm(img_a[:, :3] - R) / np.linalg.norm(R))
         ref = {"reference_spp": rs, "rel_rmse_vs_reference": round(d, 5),
                "rel_rmse_vs_reference_corrected": round(d / np.sqrt(1.0 + 1.0 / 16.0), 5)}
@@ -628,7 +637,7 @@ def workload_batch256(a) -> dict:
         single.rebuild_nmh()
         single.frame_create(s.width, s.height, s.spp)
         o = native.Context.opts(native.MODE_PBR, s.spp, s.max_depth, s.rr_start, 7 + int(k))
-        for mode, max_paths in ((0, 0), (1, 1)):
+        for mode, max_paths in ((0, 0),) if a.per_job_only else ((0, 0), (1, 1)):
             single.set_render_params(max_paths)
             if k == order[0]:
                 single.frame_render(s.camera, o)  # warm-up
@@ -646,6 +655,13 @@ def workload_batch256(a) -> dict:
     lat_plain = np.cumsum(t_single[1][order]) if len(order) else np.array([np.nan])
     log(f"per-job (A, multi-sample passes): {T_single:.3f}s total, {len(sc) / T_single:.1f} jobs/s, "
         f"{rays_single / T_single / 1e6:.0f} Mrays/s")
+    if a.per_job_only:  # profiling the per-job frames alone
+        return {"metric": "per-job frames (256 jobs)", "value": round(len(sc) / T_single, 2), "unit": "jobs/s",
+                "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+                "data": "synthetic (256 seeded Cornell scenes, 2-40 spheres)",
+                "config": {"workload": "the 256 batch jobs as one frame render each (multi-sample passes)",
+                           "per_job_multisample_s": round(T_single, 4),
+                           "per_job_mrays": round(rays_single / T_single / 1e6, 1)}}
     log(f"per-job (B, one sample per pass): {T_plain:.3f}s total, {len(sc) / T_plain:.1f} jobs/s, "
         f"{rays_single / T_plain / 1e6:.0f} Mrays/s")
 
@@ -857,6 +873,8 @@ def main() -> None:
     ap.add_argument("--spp", type=int, default=0, help="render: one render at this spp only")
     ap.add_argument("--refill", type=int, default=0, help="render: refill threshold of the persistent launches")
     ap.add_argument("--batch-only", action="store_true", help="batch256: skip the per-job side")
+    ap.add_argument("--per-job-only", action="store_true",
+                    help="batch256: only the per-job frames with multi-sample passes (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB, write MIB) 5 times: the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
