@@ -554,24 +554,42 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
       }
       break;
     }
-    for (uint32_t j = wave; j < c; j += kThreads / 64) {
-      const uint32_t o = nodeOff[j], e = nodeOff[j + 1];
-      uint32_t bk = 0, bp = 0xFFFFFFFFu;
-      for (uint32_t i = o + 1 + lane; i < e; i += 64) {
-        const uint32_t k = MK[i];
-        if (k > bk) {
-          bk = k;
-          bp = i;
+    // first maximum of the max key over each node's [o + 1, e): one wave per
+    // node while nodes are large, one thread per node once a sort group (two
+    // nodes) holds at most 64 entries
+    if (gmax > 64) {
+      for (uint32_t j = wave; j < c; j += kThreads / 64) {
+        const uint32_t o = nodeOff[j], e = nodeOff[j + 1];
+        uint32_t bk = 0, bp = 0xFFFFFFFFu;
+        for (uint32_t i = o + 1 + lane; i < e; i += 64) {
+          const uint32_t k = MK[i];
+          if (k > bk) {
+            bk = k;
+            bp = i;
+          }
         }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t ok = __shfl_xor(bk, off), op = __shfl_xor(bp, off);
-        if (ok > bk || (ok == bk && op < bp)) {
-          bk = ok;
-          bp = op;
+        for (int off = 32; off > 0; off >>= 1) {
+          const uint32_t ok = __shfl_xor(bk, off), op = __shfl_xor(bp, off);
+          if (ok > bk || (ok == bk && op < bp)) {
+            bk = ok;
+            bp = op;
+          }
         }
+        if (lane == 0) nodeArg[j] = (bp > o && bp < e) ? bp : o + 1;  // sizes are >= 2
       }
-      if (lane == 0) nodeArg[j] = (bp > o && bp < e) ? bp : o + 1;  // sizes are >= 2
+    } else {
+      for (uint32_t j = tid; j < c; j += kThreads) {
+        const uint32_t o = nodeOff[j], e = nodeOff[j + 1];
+        uint32_t bk = 0, bp = 0xFFFFFFFFu;
+        for (uint32_t i = o + 1; i < e; ++i) {
+          const uint32_t k = MK[i];
+          if (k > bk) {
+            bk = k;
+            bp = i;
+          }
+        }
+        nodeArg[j] = (bp > o && bp < e) ? bp : o + 1;
+      }
     }
     __syncthreads();
     {
@@ -594,16 +612,26 @@ __global__ __launch_bounds__(kThreads, 4) void k_subtree(const SubGroup* __restr
         if (j < c) write_pair_k(f + j, px[r], py[r], fx[r], fy[r], level, L, bb, N, P, orig);
       }
     }
-    for (uint32_t i = tid; i < m; i += kThreads) {
-      uint32_t lo = 0, hi = c;  // find j with nodeOff[j] <= i < nodeOff[j+1]
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (nodeOff[mid] <= i)
-          lo = mid;
-        else
-          hi = mid;
+    // compaction: the node of sorted position i is one of its sort group's
+    // two nodes (groups are contiguous and keep their ranges through the
+    // sort); all reads of G come before the writes
+    uint32_t nodeOf[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      uint32_t j = 0;
+      if (i < m) {
+        j = 2u * G[i];
+        if (j + 1 < c && i >= nodeOff[j + 1]) ++j;
       }
-      const uint32_t o = nodeOff[lo];
+      nodeOf[r] = j;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const uint32_t i = tid + r * kThreads;
+      if (i >= m) break;
+      const uint32_t lo = nodeOf[r], o = nodeOff[lo];
       if (i >= o + 2) {
         const uint32_t val = (i == nodeArg[lo]) ? E2[o + 1] : E2[i];
         const uint32_t dst = i - 2 * (lo + 1);
