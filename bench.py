@@ -702,6 +702,7 @@ def workload_batch256(a) -> dict:
     log(f"per-job two-seed rel-RMSE at the jobs' spp: min {job_err.min():.4f} median {np.median(job_err):.4f} "
         f"max {job_err.max():.4f}; spp for 1%: median {np.median(spp_1pct):.0f}")
     log(f"batched: {T_batch:.3f}s, {len(sc) / T_batch:.1f} jobs/s, {rays_batch / T_batch / 1e6:.0f} Mrays/s")
+    at1 = None if a.batch_only else batch_at_1pct(ctx, ids, sc, job_err)
     img = img_a
     same = 0
     for k in check if not a.batch_only else ():
@@ -743,8 +744,66 @@ def workload_batch256(a) -> dict:
                                           "max": round(float(job_err.max()), 5),
                                           "jobs_at_or_below_1pct": int((job_err <= 0.01).sum())},
             "per_job_spp_for_1pct_median": round(float(np.median(spp_1pct)), 1),
+            "at_1pct": at1,
         },
     }
+
+
+def batch_at_1pct(ctx, ids, sc, job_err, n_jobs: int = 16) -> dict:
+    """The n smallest jobs at the spp each needs for 1% rel-RMSE (from its
+    two-seed estimate at its own spp, error ~ 1/sqrt(spp), with twice that
+    spp as margin for the estimate's own noise): one batch against per-job frame renders of the same jobs and
+    seeds, and every batch image against a 4x-spp render of its job (rel-RMSE
+    with the reference's own variance, 1/4 of the image's, taken out)."""
+    import torch
+
+    from aurora_amd import native
+
+    pix = np.array([s.width * s.height for s in sc])
+    sel = [int(k) for k in np.argsort(pix, kind="stable")[:n_jobs]]
+    spp = [int(np.ceil(sc[k].spp * (job_err[k] / 0.01) ** 2 * 2.0)) for k in sel]
+    jobs = native.Context.make_jobs(
+        dict(scene=ids[k], width=sc[k].width, height=sc[k].height, spp=spp[i], max_depth=sc[k].max_depth,
+             rr_start=sc[k].rr_start, seed=500 + k, camera=sc[k].camera) for i, k in enumerate(sel))
+    out = torch.zeros((int(sum(pix[k] for k in sel)), 4), dtype=torch.float32, device="cuda")
+    ctx.batch_render_ptr(jobs, out.data_ptr())  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.batch_render_ptr(jobs, out.data_ptr())
+    torch.cuda.synchronize()
+    T_b = time.perf_counter() - t0
+    img = out.cpu().numpy()
+    single = native.Context(0)
+    T_a, errs, same = 0.0, [], 0
+    for i, k in enumerate(sel):
+        s = sc[k]
+        single.upload_scene(s)
+        single.rebuild_nmh()
+        single.frame_create(s.width, s.height, spp[i])
+        o = native.Context.opts(native.MODE_PBR, spp[i], s.max_depth, s.rr_start, 500 + k)
+        if i == 0:
+            single.frame_render(s.camera, o)  # warm-up
+        t1 = time.perf_counter()
+        single.frame_render(s.camera, o)
+        T_a += time.perf_counter() - t1
+        o0, n = int(jobs[i]["out_offset"]), int(pix[k])
+        mine = img[o0:o0 + n]
+        same += int(np.array_equal(single.frame_read(), mine))
+        single.frame_create(s.width, s.height, 4 * spp[i])
+        single.frame_render(s.camera, native.Context.opts(native.MODE_PBR, 4 * spp[i], s.max_depth, s.rr_start,
+                                                          900_000 + k))
+        R = single.frame_read()[:, :3].astype(np.float64)
+        d = float(np.linalg.norm(mine[:, :3] - R) / np.linalg.norm(R))
+        errs.append(d / np.sqrt(1.0 + 1.0 / 4.0))
+    single.close()
+    errs = np.array(errs)
+    log(f"at 1%: {len(sel)} jobs of {int(pix[sel[0]])}..{int(pix[sel[-1]])} pixels, spp {min(spp)}..{max(spp)}: "
+        f"batch {T_b:.3f}s, per-job frames {T_a:.3f}s ({T_a / T_b:.2f}x); rel-RMSE against 4x-spp references "
+        f"max {errs.max():.4f}, median {np.median(errs):.4f}; identical images {same}/{len(sel)}")
+    return {"jobs": len(sel), "job_indices": sel, "spp": spp, "batch_s": round(T_b, 4),
+            "per_job_multisample_s": round(T_a, 4), "speedup": round(T_a / T_b, 3),
+            "rel_rmse_vs_4x_reference_corrected": [round(float(e), 5) for e in errs],
+            "rel_rmse_max": round(float(errs.max()), 5), "identical_images": f"{same}/{len(sel)}"}
 
 
 def compat_frame(a, scene, label: str) -> dict:
