@@ -60,6 +60,7 @@ aur_status aur_scene_add(aur_context* ctx, uint32_t count, const float* vertices
   ctx->num_shaders = 0;
   ctx->num_lights = 0;
   ctx->num_slots = 0;
+  ctx->compat_slots = 0;
   *scene_id = (uint32_t)ctx->scenes.size();
   ctx->scenes.push_back(std::move(sc));
   return AUR_OK;

@@ -180,12 +180,13 @@ aur_status aur_geometry_upload(aur_context* ctx, uint32_t count, const float* ve
 aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader* shaders) {
   if (!ctx) return AUR_ERR_INVALID_ARG;
   if (count == 0 || !shaders) return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: empty");
-  bool textured = false, branching = false;
+  bool textured = false, branching = false, children = false;
   for (uint32_t i = 0; i < count; ++i) {
     if (shaders[i].type != 0 && shaders[i].type != 1)
       return ctx->fail(AUR_ERR_INVALID_ARG, "shaders: unknown type");
     textured = textured || shaders[i].texture != 0;
     branching = branching || (shaders[i].reflectivity > 0.0f && shaders[i].translucence > 0.0f);
+    children = children || shaders[i].reflectivity > 0.0f || shaders[i].translucence > 0.0f;
   }
   AUR_HIP(ctx, hipSetDevice(ctx->device));
   AUR_HIP(ctx, ctx->shaders.reserve(count));
@@ -193,6 +194,7 @@ aur_status aur_shaders_upload(aur_context* ctx, uint32_t count, const aur_shader
   ctx->num_shaders = count;
   ctx->textured_shaders = textured;
   ctx->branching_shaders = branching;
+  ctx->child_shaders = children;
   return AUR_OK;
 }
 
@@ -238,6 +240,9 @@ aur_status aur_lights_upload(aur_context* ctx, uint32_t count, const aur_light* 
   }
   ctx->num_lights = count;
   ctx->num_slots = aur::light_set_slots(count, lights);
+  uint64_t cslots = 0;  // compat: the estimator's shadow rays per hit
+  for (uint32_t i = 0; i < count; ++i) cslots += lights[i].type == AUR_LIGHT_AREA ? (uint64_t)lights[i].samples : 1u;
+  ctx->compat_slots = cslots > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cslots;
   return AUR_OK;
 }
 

@@ -153,6 +153,13 @@ struct FrameState {
   DevBuf<uint32_t> sh_queue;  // slots holding a shadow ray, appended by shading
   DevBuf<float2> gstack;      // traversal stack rows beyond the LDS window
   DevBuf<float4> compat_pend; // compat mode: mirror rays waiting below a refraction ray
+  // compat wavefront, per (pixel, sample) entry of a round: the stack's top
+  // ray (origin + weight, direction + eta), colour + last hit, (depth, rays
+  // waiting, random draws, frame pixel), final colour; per queue position:
+  // emission + ray weight, albedo + flags; per (shadow slot, queue
+  // position): contribution + cast flag
+  DevBuf<float4> cw_o, cw_d, cw_c, cw_emw, cw_c1, cw_contrib, cw_fin;
+  DevBuf<uint4> cw_i;
   DevBuf<uint32_t> sh_job;    // batches: job of each shadow-queue entry
   // adjoint replay: path-vertex records, per-block partial gradients,
   // running gradient sums and the largest |dLoss/dI|
@@ -209,6 +216,8 @@ struct aur_context {
   uint32_t num_slots = 0;  // next-event slots per path vertex
   bool textured_shaders = false;  // a shader uses a texture map
   bool branching_shaders = false;  // a shader is both reflective and translucent (two compat child rays)
+  bool child_shaders = false;      // a shader is reflective or translucent (compat child rays)
+  uint32_t compat_slots = 0;       // compat shadow rays per hit: area-light samples, one per other light
   // RGBA8 textures (aur_textures_upload)
   aur::DevBuf<aur::TexDesc> textures;
   aur::DevBuf<uint8_t> texels;

@@ -3,12 +3,14 @@
 // (cudaGenerateRays, utility.cu:18-94; cudaRaytraceMonteCarlo,
 // monteCarloRaytracer.cu:20-124; cudaDrawPixels, utility.cu:199-229).
 //
-// AUR_MODE_COMPAT reproduces the reference's Monte Carlo ray tracer: one
-// thread per pixel sample walks the reference's ray stack (reflection and
-// refraction only, MAX_DEPTH 8), with the reference's direct-light
-// estimators (radiance.cuh:8-80, light.cuh:24-60) and shadow rays through
-// the shared traversal.  cuRAND is replaced by Philox keyed by
-// (seed, pixel, sample, draw index).
+// AUR_MODE_COMPAT reproduces the reference's Monte Carlo ray tracer: each
+// pixel sample walks the reference's ray stack (reflection and refraction
+// only, MAX_DEPTH 8) with the reference's direct-light estimators
+// (radiance.cuh:8-80, light.cuh:24-60), as wavefront passes: per stack
+// level, closest hit -> shade (the estimators' draws and shadow rays, child
+// rays) -> shadow (any hit) -> resolve (the estimators' sums in light and
+// sample order).  cuRAND is replaced by Philox keyed by (seed, pixel,
+// sample, draw index).
 //
 // AUR_MODE_PBR is a wavefront path loop: per sample pass, camera rays ->
 // [extend (closest hit) -> shade (emission, one next-event slot per light
@@ -76,13 +78,6 @@ __device__ inline V3 interp9(const float* base, uint32_t tri, float u, float v) 
   return bclerp(mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), mk(p[6], p[7], p[8]), u, v);
 }
 
-// Wave-aggregated 64-bit counter increment.
-__device__ inline void count_add(unsigned long long* c, uint32_t v) {
-  uint32_t s = v;
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if ((threadIdx.x & 63u) == 0 && s) atomicAdd(c, (unsigned long long)s);
-}
-
 // ---------------------------------------------------------------- compat --
 
 struct CompatRng {
@@ -118,181 +113,14 @@ struct CRay {
   float t, weight, eta;
 };
 
-__device__ inline bool occluded(const SceneDev& sc, V3 o, V3 d, float tmax, float2* stk,
-                                uint32_t stride, uint32_t& nrays) {
-  ++nrays;
-  return trace_ray<true>(sc.nodes, sc.n_nodes, sc.rec_bytes, o, d, tmax, stk, stride).id >= 0;
-}
-
 // The reference's ray stack (Stack<Ray, MAX_DEPTH + 2>, popped last-in
 // first-out) without a dynamically indexed per-lane array: the ray on top of
-// the stack stays in registers, and a hit pushes a ray into memory only when
-// it spawns two (mirror and refraction): the mirror ray, pushed first, waits
-// in `pend` below the refraction ray, which becomes the top.  Each such hit
-// raises depth, and a child is pushed only while depth <= 8, so at most
-// kCompatPend rays wait.  pend rows are [row][lane] (two float4 per entry:
-// origin + weight, direction + eta), `pend_stride` lanes apart; null when no
-// shader is both reflective and translucent.
+// the stack is kept with the pixel's state, and a hit pushes a ray into
+// memory only when it spawns two (mirror and refraction): the mirror ray,
+// pushed first, waits in `pend` below the refraction ray, which becomes the
+// top.  Each such hit raises depth, and a child is pushed only while depth
+// <= 8, so at most kCompatPend rays wait.
 constexpr int kCompatPend = 8;
-
-// region: (left, bottom, region width, region pixels); thread l renders
-// region pixel l, frame pixel (bottom + l / width) * w + left + l % width.
-__global__ __launch_bounds__(kBlock) void k_compat(SceneDev sc, aur_camera cam, uint32_t w,
-                                                   uint32_t h, uint4 region, uint32_t spp,
-                                                   uint32_t sample, uint32_t seed,
-                                                   float4* __restrict__ pixels,
-                                                   unsigned long long* __restrict__ nrays_total,
-                                                   float4* __restrict__ pend, uint32_t pend_stride) {
-  extern __shared__ float2 stk_mem[];
-  float2* stk = stk_mem + threadIdx.x;
-  const uint32_t stride = blockDim.x;
-  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nrays = 0;
-  if (l < region.w) {
-    const uint32_t x = region.x + l % region.z, y = region.y + l / region.z;
-    const uint32_t pix = y * w + x;
-    CompatRng rng{seed, pix, sample, 0u, U4{0, 0, 0, 0}};
-    const int kMaxDepth = 8;
-    float4* mine = pend + 2 * (size_t)l;
-    const size_t row = 2 * (size_t)pend_stride;
-    int np = 0;  // rays waiting in pend below the top
-    CRay top{ld3(cam.position), camera_dir_compat(cam, w, h, x, y, sample, spp), kInfinity, 1.0f, 1.0f};
-    bool have = true;  // the stack holds `top` (and the np rays below it)
-    auto pop_pending = [&]() {
-      if (np == 0) return false;
-      --np;
-      const float4 a = mine[np * row], b = mine[np * row + 1];
-      top = CRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), kInfinity, a.w, b.w};
-      return true;
-    };
-    int depth = 0;
-    V3 color = mk(0.0f);
-    int32_t hpId = -1;
-    float hpU = 0.0f, hpV = 0.0f;
-    while (depth <= kMaxDepth && have) {
-      CRay ray = top;
-      have = false;
-      ++nrays;
-      const TraceResult hr =
-          trace_ray<false>(sc.nodes, sc.n_nodes, sc.rec_bytes, ray.pos, ray.dir, ray.t, stk, stride);
-      if (hr.id < 0) {
-        have = pop_pending();
-        continue;
-      }
-      ray.t = hr.t;
-      hpId = hr.id;
-      hpU = hr.u;
-      hpV = hr.v;
-      const V3 position = ray.pos + (ray.dir * ray.t);
-      const V3 wo = -ray.dir;
-      const aur_shader& sh = shader_of(sc, (uint32_t)hpId);
-      const V3 N = normalize(interp9(sc.normals, hpId, hpU, hpV));
-      V3 color1 = ld3(sh.diffuse_color) * sh.diffuse;
-      if (sh.texture > 0)
-        color1 = tex_color(sc.texcoords, sc.textures, sc.texels, sc.num_textures, sh.texture, (uint32_t)hpId,
-                           hpU, hpV) * sh.diffuse;
-      V3 Li = ld3(sh.emission_color);
-      for (uint32_t i = 0; i < sc.num_lights; ++i) {
-        const aur_light& L = sc.lights[i];
-        const V3 CI = ld3(L.color) * L.intensity;
-        if (L.type == AUR_LIGHT_AMBIENT) Li = Li + (color1 * kInvPi) * (ld3(L.color) * L.intensity * L.ambient);
-        if (L.type != AUR_LIGHT_AREA) {
-          // estimateDirectRadianceDelta (radiance.cuh:13-28)
-          V3 wdir;
-          float wt;
-          V3 Lv;
-          if (L.type == AUR_LIGHT_AMBIENT) {
-            wdir = ld3(L.position) - position;
-            wt = length(wdir);
-            wdir = wdir / wt;
-            Lv = CI * (1.0f - L.ambient);
-          } else if (L.type == AUR_LIGHT_POINT) {
-            wdir = ld3(L.position) - position;
-            wt = length(wdir);
-            wdir = wdir / wt;
-            Lv = CI / (wt * wt);
-          } else {
-            wdir = -ld3(L.direction);
-            wt = kInfinity;
-            Lv = CI;
-          }
-          V3 add = mk(0.0f);
-          if (!is_zero(Lv)) {
-            const V3 f = color1 * kInvPi;
-            if (!is_zero(f) && !occluded(sc, position, wdir, wt, stk, stride, nrays))
-              add = f * Lv * fmaxf(0.0f, dot(wdir, N));
-          }
-          Li = Li + add;
-        } else {
-          // estimateDirectRadiance (radiance.cuh:30-80)
-          V3 acc = mk(0.0f);
-          for (int s = 0; s < L.samples; ++s) {
-            V3 Ls = mk(0.0f);
-            const float u1 = rng.next();
-            const float u2 = rng.next();
-            const V3 Pl = ld3(L.position) + ((ld3(L.e1) * u1) + (ld3(L.e2) * u2));
-            V3 wdir = Pl - position;
-            float wt = length(wdir);
-            wdir = wdir / wt;
-            const float lightPdf = 1.0f / fabsf(dot(ld3(L.direction), -wdir));
-            const V3 Lv = CI * fmaxf(0.0f, dot(ld3(L.direction), -wdir));
-            V3 wpos = position;
-            wpos.x += kEpsilon * wdir.x;
-            wpos.y += kEpsilon * wdir.y;
-            wpos.z += kEpsilon * wdir.z;
-            wt += kEpsilon;
-            if (lightPdf > 0.0f && !is_zero(Lv)) {
-              const V3 f = color1 * kInvPi;
-              if (!is_zero(f) && !occluded(sc, wpos, wdir, wt, stk, stride, nrays))
-                Ls = Ls + f * Lv * fmaxf(0.0f, dot(wdir, N));
-            }
-            acc = acc + Ls;
-          }
-          const float nsf = (float)L.samples;
-          Li = Li + mk(acc.x / nsf, acc.y / nsf, acc.z / nsf);
-        }
-      }
-      if (sh.reflectivity > 0.0f || sh.translucence > 0.0f) depth++;
-      if (depth <= kMaxDepth && sh.reflectivity > 0.0f) {
-        top = CRay{position, reflect(ray.dir, N), kInfinity, sh.reflectivity, ray.eta};
-        ray.weight *= (1.0f - sh.reflectivity);
-        have = true;
-      }
-      if (depth <= kMaxDepth && sh.translucence > 0.0f) {
-        V3 rN = N;
-        float etai = ray.eta, etat = sh.refractive_index;
-        if (dot(rN, wo) < 0.0f) {
-          rN = -rN;
-          etai = sh.refractive_index;
-          etat = 1.0f;
-        }
-        V3 tdir;
-        if (refract(tdir, ray.dir, rN, etai, etat)) {
-          if (have) {  // the mirror ray waits below the refraction ray
-            mine[np * row] = make_float4(top.pos.x, top.pos.y, top.pos.z, top.weight);
-            mine[np * row + 1] = make_float4(top.dir.x, top.dir.y, top.dir.z, top.eta);
-            ++np;
-          }
-          top = CRay{position, tdir, kInfinity, sh.translucence, etat};
-          ray.weight *= (1.0f - sh.translucence);
-          have = true;
-        }
-      }
-      if (!have) have = pop_pending();
-      color = color + (Li * ray.weight);
-    }
-    if (hpId != -1) {
-      const float weight = 1.0f / (float)spp;
-      float4 px = pixels[pix];
-      px.x += color.x * weight;
-      px.y += color.y * weight;
-      px.z += color.z * weight;
-      px.w += weight;
-      pixels[pix] = px;
-    }
-  }
-  count_add(nrays_total, nrays);
-}
 
 // ------------------------------------------------------------------- PBR --
 
@@ -1012,6 +840,340 @@ uint32_t resident_grid(aur_context* ctx, const void* fn, size_t lds, uint64_t ma
   return (uint32_t)(blocks ? blocks : 1);
 }
 
+// ------------------------------------------------------- compat wavefront --
+//
+// Compat mode as wavefront passes over rounds of (pixel, sample) entries:
+// k_cw_init sets up each entry's stack with its camera ray; then per stack
+// level a closest-hit launch over the entries still active, k_cw_shade (one
+// ray of each entry's stack: the direct-light estimators with their random
+// draws, shadow rays appended to a queue, child rays pushed), an any-hit
+// launch over the shadow queue and k_cw_resolve (the estimators' sums in
+// light and sample order from the shadow results, colour += Li * weight,
+// then the entry continues or keeps its final colour); k_cw_accumulate then
+// adds each pixel's samples in sample order, as the reference's launch per
+// sample does.  Per entry, the rays, draws and sums are those of the
+// reference's loop over its stack (monteCarloRaytracer.cu:54-85,
+// radiance.cuh:13-80) in the same order; the persistent traversal launches
+// refill lanes between node visits.
+struct CwArgs {
+  SceneDev sc;
+  aur_camera cam;
+  uint32_t w, h, spp, seed;
+  uint4 region;     // (left, bottom, region width, region pixels)
+  uint32_t l0;      // first region pixel of the chunk
+  uint32_t nchunk;  // pixels in the chunk
+  uint32_t s0;      // first sample of the round: entry l is chunk pixel l % nchunk, sample s0 + l / nchunk
+  uint32_t n;       // entries of the round
+  uint32_t stride;  // entry capacity: stride of the per-slot arrays
+  uint32_t nslots;  // shadow slots per hit
+  float4 *st_o, *st_d, *st_c;
+  uint4* st_i;
+  float4* pend;  // rays waiting below the top: [row][chunk pixel], two float4 each
+  float4 *emw, *c1, *contrib;
+  float4* fin;  // per entry: final colour, 1 when a ray of the stack hit (else 0)
+  float4* pixels;
+};
+
+constexpr uint32_t kCwHit = 1u, kCwMore = 2u;  // c1.w flags: the ray hit; the pixel continues
+
+// Exclusive prefix of n over the wave's lanes plus one atomic per wave on
+// *counter: the position of this lane's first of n appended entries.  Every
+// lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t n) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t incl = n;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += t;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, 0);
+  return base + incl - n;
+}
+
+__device__ __forceinline__ CompatRng compat_rng(uint32_t seed, uint32_t pix, uint32_t sample, uint32_t k) {
+  CompatRng r{seed, pix, sample, k, U4{0, 0, 0, 0}};
+  if (k & 3u) r.cache = philox(U4{pix, sample, 0x40000000u | (k >> 2), 0u}, seed, kKeyCompat);
+  return r;
+}
+
+__global__ void k_cw_init(CwArgs c, PbrArgs a, uint32_t* __restrict__ qin) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l == 0) {
+    a.counters[0] = c.n;
+    a.counters[1] = 0;
+  }
+  if (l >= c.n) return;
+  const uint32_t rl = c.l0 + l % c.nchunk;
+  const uint32_t x = c.region.x + rl % c.region.z, y = c.region.y + rl / c.region.z;
+  const V3 o = ld3(c.cam.position);
+  const V3 d = camera_dir_compat(c.cam, c.w, c.h, x, y, c.s0 + l / c.nchunk, c.spp);
+  c.st_o[l] = make_float4(o.x, o.y, o.z, 1.0f);
+  c.st_d[l] = make_float4(d.x, d.y, d.z, 1.0f);
+  c.st_c[l] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+  c.st_i[l] = make_uint4(0u, 0u, 0u, y * c.w + x);
+  a.ray_o[l] = make_float4(o.x, o.y, o.z, 0.0f);
+  a.ray_d[l] = make_float4(d.x, d.y, d.z, 0.0f);
+  qin[l] = l;
+}
+
+// One ray of each active pixel's stack, up to the shadow rays.
+__global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32_t w,
+                                                     const uint32_t* __restrict__ qin) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = a.counters[2 * w];
+  const bool act = i < n;
+  const int kMaxDepth = 8;
+  uint32_t ncast = 0;
+  if (act) {
+    const uint32_t l = qin[i];
+    const float4 so = c.st_o[l], sd = c.st_d[l], sc4 = c.st_c[l];
+    uint4 si = c.st_i[l];
+    int depth = (int)si.x, np = (int)si.y;
+    const uint32_t pix = si.w;
+    int32_t hpId = __float_as_int(sc4.w);
+    float4* mine = c.pend + 2 * (size_t)l;
+    const size_t row = 2 * (size_t)c.stride;
+    CRay top{mk(so.x, so.y, so.z), mk(sd.x, sd.y, sd.z), kInfinity, so.w, sd.w};
+    auto pop_pending = [&]() {
+      if (np == 0) return false;
+      --np;
+      const float4 pa = mine[np * row], pb = mine[np * row + 1];
+      top = CRay{mk(pa.x, pa.y, pa.z), mk(pb.x, pb.y, pb.z), kInfinity, pa.w, pb.w};
+      return true;
+    };
+    CRay ray = top;
+    bool have = false;
+    uint32_t flags = 0;
+    float4 c1v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const aur_hit hr = a.hits[i];
+    if (hr.id < 0) {
+      have = pop_pending();
+    } else {
+      flags = kCwHit;
+      ray.t = hr.t;
+      hpId = hr.id;
+      const V3 position = ray.pos + (ray.dir * ray.t);
+      const V3 wo = -ray.dir;
+      const aur_shader& sh = shader_of(c.sc, (uint32_t)hpId);
+      const V3 N = normalize(interp9(c.sc.normals, hpId, hr.u, hr.v));
+      V3 color1 = ld3(sh.diffuse_color) * sh.diffuse;
+      if (sh.texture > 0)
+        color1 = tex_color(c.sc.texcoords, c.sc.textures, c.sc.texels, c.sc.num_textures, sh.texture,
+                           (uint32_t)hpId, hr.u, hr.v) * sh.diffuse;
+      const V3 f = color1 * kInvPi;
+      CompatRng rng = compat_rng(c.seed, pix, c.s0 + l / c.nchunk, si.z);
+      uint32_t slot = 0;
+      // a shadow ray of slot `slot` (not queued yet) and its contribution
+      // when unoccluded; a slot without a ray contributes zero
+      const auto put = [&](bool cast, V3 o, V3 d, float tmax, V3 val) {
+        const size_t e = (size_t)slot * c.stride + i;
+        c.contrib[e] = make_float4(val.x, val.y, val.z, cast ? 1.0f : 0.0f);
+        if (cast) {
+          a.sh_o[e] = make_float4(o.x, o.y, o.z, tmax);
+          a.sh_d[e] = make_float4(d.x, d.y, d.z, 0.0f);
+          ++ncast;
+        }
+        ++slot;
+      };
+      for (uint32_t li = 0; li < c.sc.num_lights; ++li) {
+        const aur_light& L = c.sc.lights[li];
+        const V3 CI = ld3(L.color) * L.intensity;
+        if (L.type != AUR_LIGHT_AREA) {
+          // estimateDirectRadianceDelta (radiance.cuh:13-28)
+          V3 wdir;
+          float wt;
+          V3 Lv;
+          if (L.type == AUR_LIGHT_AMBIENT) {
+            wdir = ld3(L.position) - position;
+            wt = length(wdir);
+            wdir = wdir / wt;
+            Lv = CI * (1.0f - L.ambient);
+          } else if (L.type == AUR_LIGHT_POINT) {
+            wdir = ld3(L.position) - position;
+            wt = length(wdir);
+            wdir = wdir / wt;
+            Lv = CI / (wt * wt);
+          } else {
+            wdir = -ld3(L.direction);
+            wt = kInfinity;
+            Lv = CI;
+          }
+          const bool cast = !is_zero(Lv) && !is_zero(f);
+          put(cast, position, wdir, wt, cast ? f * Lv * fmaxf(0.0f, dot(wdir, N)) : mk(0.0f));
+        } else {
+          // estimateDirectRadiance (radiance.cuh:30-80)
+          for (int s = 0; s < L.samples; ++s) {
+            const float u1 = rng.next();
+            const float u2 = rng.next();
+            const V3 Pl = ld3(L.position) + ((ld3(L.e1) * u1) + (ld3(L.e2) * u2));
+            V3 wdir = Pl - position;
+            float wt = length(wdir);
+            wdir = wdir / wt;
+            const float lightPdf = 1.0f / fabsf(dot(ld3(L.direction), -wdir));
+            const V3 Lv = CI * fmaxf(0.0f, dot(ld3(L.direction), -wdir));
+            V3 wpos = position;
+            wpos.x += kEpsilon * wdir.x;
+            wpos.y += kEpsilon * wdir.y;
+            wpos.z += kEpsilon * wdir.z;
+            wt += kEpsilon;
+            const bool cast = lightPdf > 0.0f && !is_zero(Lv) && !is_zero(f);
+            put(cast, wpos, wdir, wt, cast ? f * Lv * fmaxf(0.0f, dot(wdir, N)) : mk(0.0f));
+          }
+        }
+      }
+      si.z = rng.k;
+      if (sh.reflectivity > 0.0f || sh.translucence > 0.0f) depth++;
+      if (depth <= kMaxDepth && sh.reflectivity > 0.0f) {
+        top = CRay{position, reflect(ray.dir, N), kInfinity, sh.reflectivity, ray.eta};
+        ray.weight *= (1.0f - sh.reflectivity);
+        have = true;
+      }
+      if (depth <= kMaxDepth && sh.translucence > 0.0f) {
+        V3 rN = N;
+        float etai = ray.eta, etat = sh.refractive_index;
+        if (dot(rN, wo) < 0.0f) {
+          rN = -rN;
+          etai = sh.refractive_index;
+          etat = 1.0f;
+        }
+        V3 tdir;
+        if (refract(tdir, ray.dir, rN, etai, etat)) {
+          if (have) {  // the mirror ray waits below the refraction ray
+            mine[np * row] = make_float4(top.pos.x, top.pos.y, top.pos.z, top.weight);
+            mine[np * row + 1] = make_float4(top.dir.x, top.dir.y, top.dir.z, top.eta);
+            ++np;
+          }
+          top = CRay{position, tdir, kInfinity, sh.translucence, etat};
+          ray.weight *= (1.0f - sh.translucence);
+          have = true;
+        }
+      }
+      if (!have) have = pop_pending();
+      const V3 em = ld3(sh.emission_color);
+      c.emw[i] = make_float4(em.x, em.y, em.z, ray.weight);
+      c1v = make_float4(color1.x, color1.y, color1.z, 0.0f);
+    }
+    if (depth <= kMaxDepth && have) flags |= kCwMore;
+    c1v.w = __uint_as_float(flags);
+    c.c1[i] = c1v;
+    c.st_o[l] = make_float4(top.pos.x, top.pos.y, top.pos.z, top.weight);
+    c.st_d[l] = make_float4(top.dir.x, top.dir.y, top.dir.z, top.eta);
+    c.st_c[l].w = __int_as_float(hpId);
+    si.x = (uint32_t)depth;
+    si.y = (uint32_t)np;
+    c.st_i[l] = si;
+  }
+  // the cast slots enter the shadow queue slot by slot, each slot's rays in
+  // lane order (neighbouring pixels' rays toward the same light sample stay
+  // together for the traversal); one atomic per wave
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t total = 0;
+  for (uint32_t slot = 0; slot < c.nslots; ++slot)
+    total += (uint32_t)__popcll(__ballot(act && ncast && c.contrib[(size_t)slot * c.stride + i].w != 0.0f));
+  uint32_t run = 0;
+  if (lane == 0 && total) run = atomicAdd(&a.counters[2 * w + 1], total);
+  run = __shfl(run, 0);
+  for (uint32_t slot = 0; slot < c.nslots && total; ++slot) {
+    const size_t e = (size_t)slot * c.stride + i;
+    const bool cast = act && ncast && c.contrib[e].w != 0.0f;
+    const uint64_t m = __ballot(cast);
+    if (cast) a.sh_queue[run + (uint32_t)__popcll(m & lt)] = (uint32_t)e;
+    run += (uint32_t)__popcll(m);
+  }
+}
+
+// The estimators' sums from the shadow results, then the next stack ray or
+// the pixel's sample.
+// Each pixel of the chunk: its samples of the round in sample order
+// (cudaDrawPixels, utility.cu:199-229: colour / spp and coverage alpha when a
+// ray of the sample hit).
+__global__ void k_cw_accumulate(CwArgs c, uint32_t ns) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.nchunk) return;
+  const uint32_t rl = c.l0 + p;
+  const uint32_t pix = (c.region.y + rl / c.region.z) * c.w + c.region.x + rl % c.region.z;
+  const float weight = 1.0f / (float)c.spp;
+  float4 px = c.pixels[pix];
+  for (uint32_t k = 0; k < ns; ++k) {
+    const float4 f = c.fin[(size_t)k * c.nchunk + p];
+    if (f.w != 0.0f) {
+      px.x += f.x * weight;
+      px.y += f.y * weight;
+      px.z += f.z * weight;
+      px.w += weight;
+    }
+  }
+  c.pixels[pix] = px;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cw_resolve(CwArgs c, PbrArgs a, uint32_t w, uint32_t wn,
+                                                       const uint32_t* __restrict__ qin,
+                                                       uint32_t* __restrict__ qout, uint32_t roff_next) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = a.counters[2 * w];
+  const bool act = i < n;
+  bool more = false;
+  uint32_t l = 0;
+  if (act) {
+    l = qin[i];
+    const float4 c1 = c.c1[i];
+    const uint32_t flags = __float_as_uint(c1.w);
+    float4 sc4 = c.st_c[l];
+    V3 color = mk(sc4.x, sc4.y, sc4.z);
+    if (flags & kCwHit) {
+      const float4 emw = c.emw[i];
+      const V3 color1 = mk(c1.x, c1.y, c1.z);
+      V3 Li = mk(emw.x, emw.y, emw.z);
+      uint32_t slot = 0;
+      const auto lit = [&](V3& Ls) {  // Ls + the slot's contribution when unoccluded
+        const size_t e = (size_t)slot * c.stride + i;
+        const float4 cv = c.contrib[e];
+        if (cv.w != 0.0f && a.sh_occ[e] == 0) Ls = Ls + mk(cv.x, cv.y, cv.z);
+        ++slot;
+      };
+      for (uint32_t li = 0; li < c.sc.num_lights; ++li) {
+        const aur_light& L = c.sc.lights[li];
+        if (L.type == AUR_LIGHT_AMBIENT) Li = Li + (color1 * kInvPi) * (ld3(L.color) * L.intensity * L.ambient);
+        if (L.type != AUR_LIGHT_AREA) {
+          const size_t e = (size_t)slot * c.stride + i;
+          const float4 cv = c.contrib[e];
+          V3 add = mk(0.0f);
+          if (cv.w != 0.0f && a.sh_occ[e] == 0) add = mk(cv.x, cv.y, cv.z);
+          ++slot;
+          Li = Li + add;
+        } else {
+          V3 acc = mk(0.0f);
+          for (int s = 0; s < L.samples; ++s) {
+            V3 Ls = mk(0.0f);
+            lit(Ls);
+            acc = acc + Ls;
+          }
+          const float nsf = (float)L.samples;
+          Li = Li + mk(acc.x / nsf, acc.y / nsf, acc.z / nsf);
+        }
+      }
+      color = color + (Li * emw.w);
+    }
+    more = (flags & kCwMore) != 0u;
+    if (more)
+      c.st_c[l] = make_float4(color.x, color.y, color.z, sc4.w);
+    else
+      c.fin[l] = make_float4(color.x, color.y, color.z, __float_as_int(sc4.w) != -1 ? 1.0f : 0.0f);
+  }
+  const uint32_t p = wave_append(&a.counters[2 * wn], more ? 1u : 0u);
+  if (more) {
+    const float4 so = c.st_o[l], sd = c.st_d[l];
+    qout[p] = l;
+    a.ray_o[roff_next + p] = make_float4(so.x, so.y, so.z, 0.0f);
+    a.ray_d[roff_next + p] = make_float4(sd.x, sd.y, sd.z, 0.0f);
+  }
+}
+
 constexpr uint32_t kAdjMaxDepth = 16;
 constexpr uint32_t kAdjMaxParams = 6144;  // 48 KiB of LDS accumulators
 
@@ -1047,6 +1209,33 @@ aur_status pbr_reserve(aur_context* ctx, uint32_t cap, uint32_t nslots, uint32_t
     AUR_HIP(ctx, fr.rec_mat.reserve((size_t)cap * max_depth));
     AUR_HIP(ctx, fr.rec_rr.reserve((size_t)cap * max_depth));
     AUR_HIP(ctx, fr.rec_w.reserve(std::max<size_t>(1, (size_t)cap * max_depth * nslots)));
+  }
+  return AUR_OK;
+}
+
+// Launch shape of k_pbr_trace for up to max_items rays: the traversal stack
+// takes at most levels - 1 rows per lane, the first `rows` in LDS, the rest
+// (with a window) in global memory; the grid is the resident one.
+template <bool kMulti>
+aur_status trace_shape(aur_context* ctx, uint32_t levels, uint64_t max_items, PbrArgs& a, size_t& lds,
+                       uint32_t& grid) {
+  FrameState& fr = ctx->frame;
+  const uint32_t maxrows = levels > 1 ? levels - 1 : 1;
+  const uint32_t want = ctx->render_params.stack_lds_rows;
+  const uint32_t rows = want >= maxrows ? maxrows : want;
+  lds = (size_t)rows * kBlock * sizeof(float2);
+  grid = resident_grid(ctx, rows < maxrows ? (const void*)k_pbr_trace<kMulti, true> : (const void*)k_pbr_trace<kMulti, false>,
+                       lds, max_items);
+  a.gstack = nullptr;
+  a.klds = rows;
+  a.gstack_rows = 0;
+  if (rows < maxrows) {
+    const size_t entries = (size_t)(maxrows - rows) * grid * kBlock;
+    if (entries * sizeof(float2) >= (1ull << 31))
+      return ctx->fail(AUR_ERR_INVALID_ARG, "render: global stack rows exceed 2 GiB");
+    AUR_HIP(ctx, fr.gstack.reserve(entries));
+    a.gstack = fr.gstack.p;
+    a.gstack_rows = maxrows - rows;
   }
   return AUR_OK;
 }
@@ -1155,26 +1344,10 @@ aur_status pbr_passes(aur_context* ctx, const PbrRun& run, unsigned long long* s
     a.rec_rr = fr.rec_rr.p;
     a.rec_w = fr.rec_w.p;
   }
-  // traversal stack: at most levels - 1 rows per lane; the first `rows` in
-  // LDS, the rest (with a window) in global memory
-  const uint32_t maxrows = run.levels > 1 ? run.levels - 1 : 1;
-  const uint32_t want = ctx->render_params.stack_lds_rows;
-  const uint32_t rows = want >= maxrows ? maxrows : want;
-  const size_t lds = (size_t)rows * kBlock * sizeof(float2);
-  const uint32_t grid_trace =
-      resident_grid(ctx, rows < maxrows ? (const void*)k_pbr_trace<kMulti, true> : (const void*)k_pbr_trace<kMulti, false>,
-                    lds, (uint64_t)cap * (1u + run.nslots));
-  a.gstack = nullptr;
-  a.klds = rows;
-  a.gstack_rows = 0;
-  if (rows < maxrows) {
-    const size_t entries = (size_t)(maxrows - rows) * grid_trace * kBlock;
-    if (entries * sizeof(float2) >= (1ull << 31))
-      return ctx->fail(AUR_ERR_INVALID_ARG, "render: global stack rows exceed 2 GiB");
-    AUR_HIP(ctx, fr.gstack.reserve(entries));
-    a.gstack = fr.gstack.p;
-    a.gstack_rows = maxrows - rows;
-  }
+  size_t lds = 0;
+  uint32_t grid_trace = 0;
+  st = trace_shape<kMulti>(ctx, run.levels, (uint64_t)cap * (1u + run.nslots), a, lds, grid_trace);
+  if (st != AUR_OK) return st;
   uint32_t sweep_grid = 0;
   if (adj) {
     // a few resident blocks per CU; each keeps LDS accumulators for all parameters
@@ -1283,6 +1456,129 @@ JobDev frame_job(aur_context* ctx, const aur_camera* cam, const aur_render_opts*
   return J;
 }
 
+// Compat-mode samples [s0, s1) of the frame region as wavefront passes
+// (k_cw_* above): rounds of up to max_paths (pixel, sample) entries, every
+// pixel of the region with as many whole samples as fit, or chunks of
+// pixels with one sample when the region alone is larger.
+aur_status compat_passes(aur_context* ctx, const aur_camera* cam, const aur_render_opts* o, uint32_t s0,
+                         uint32_t s1, unsigned long long* stats) {
+  FrameState& fr = ctx->frame;
+  hipStream_t s = ctx->stream;
+  const uint32_t rw = fr.right - fr.left + 1, npix = rw * (fr.top - fr.bottom + 1);
+  const uint32_t nslots = ctx->compat_slots;
+  const uint64_t per_pixel = 1ull + nslots;
+  const uint64_t budget = std::min<uint64_t>(std::max<uint32_t>(ctx->render_params.max_paths, 1u),
+                                             0xFFFFFFFFull / std::max<uint32_t>(nslots, 1u));
+  const uint32_t pix_cap = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(npix, budget));
+  const uint32_t ns_max = pix_cap == npix ? (uint32_t)std::max<uint64_t>(1u, budget / npix) : 1u;
+  const uint32_t cap = pix_cap * ns_max;  // entries per round
+  aur_status st = pbr_reserve(ctx, cap, nslots, 1, false);
+  if (st != AUR_OK) return st;
+  AUR_HIP(ctx, fr.cw_o.reserve(cap));
+  AUR_HIP(ctx, fr.cw_d.reserve(cap));
+  AUR_HIP(ctx, fr.cw_c.reserve(cap));
+  AUR_HIP(ctx, fr.cw_i.reserve(cap));
+  AUR_HIP(ctx, fr.cw_emw.reserve(cap));
+  AUR_HIP(ctx, fr.cw_c1.reserve(cap));
+  AUR_HIP(ctx, fr.cw_contrib.reserve(std::max<size_t>(1, (size_t)cap * nslots)));
+  AUR_HIP(ctx, fr.cw_fin.reserve(cap));
+  // waiting mirror rays exist only under shaders that are both reflective
+  // and translucent
+  float4* pend = nullptr;
+  if (ctx->branching_shaders) {
+    AUR_HIP(ctx, fr.compat_pend.reserve((size_t)kCompatPend * 2 * cap));
+    pend = fr.compat_pend.p;
+  }
+  PbrArgs a{};
+  a.job = frame_job(ctx, cam, o);
+  a.cap = cap;
+  a.nslots = nslots;
+  a.max_depth = 1;
+  a.refill = ctx->render_params.refill;
+  a.ray_o = fr.ray_o.p;
+  a.ray_d = fr.ray_d.p;
+  a.hits = fr.hits.p;
+  a.sh_o = fr.sh_o.p;
+  a.sh_d = fr.sh_d.p;
+  a.sh_occ = fr.sh_occ.p;
+  a.sh_queue = fr.sh_queue.p;
+  a.counters = fr.counters.p;
+  a.stats = stats;
+  size_t lds = 0;
+  uint32_t grid_trace = 0;
+  st = trace_shape<false>(ctx, ctx->levels, (uint64_t)cap * per_pixel, a, lds, grid_trace);
+  if (st != AUR_OK) return st;
+  CwArgs c{};
+  c.sc = SceneDev{ctx->nodes.p,      ctx->count / 2,   ctx->rec_bytes,   ctx->normals.p,
+                  ctx->shader_ids.p, ctx->shaders.p,   ctx->num_shaders, ctx->lights.p,
+                  ctx->num_lights,   ctx->texcoords.p, ctx->textures.p,  ctx->texels.p,
+                  ctx->num_textures};
+  c.cam = *cam;
+  c.w = fr.width;
+  c.h = fr.height;
+  c.spp = o->spp;
+  c.seed = o->seed;
+  c.region = make_uint4(fr.left, fr.bottom, rw, npix);
+  c.stride = cap;
+  c.nslots = nslots;
+  c.st_o = fr.cw_o.p;
+  c.st_d = fr.cw_d.p;
+  c.st_c = fr.cw_c.p;
+  c.st_i = fr.cw_i.p;
+  c.pend = pend;
+  c.emw = fr.cw_emw.p;
+  c.c1 = fr.cw_c1.p;
+  c.contrib = fr.cw_contrib.p;
+  c.fin = fr.cw_fin.p;
+  c.pixels = fr.pixels.p;
+  const auto trace = [&](uint32_t w, uint32_t parts, uint32_t roff) {
+    if (a.gstack)
+      k_pbr_trace<false, true><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff);
+    else
+      k_pbr_trace<false, false><<<grid_trace, kBlock, lds, s>>>(a, w, parts, roff);
+  };
+  // every stack ray is a hit of depth <= 8 or below one: at most 1 + 2 * 9
+  // rays per pixel sample, so the stacks empty within that many levels
+  constexpr uint32_t kMaxLevels = 19;
+  PassThrottle throttle(s);
+  uint32_t round = 0;
+  for (uint32_t sa = s0; sa < s1; sa += ns_max) {
+    const uint32_t ns = std::min(ns_max, s1 - sa);
+    AUR_HIP(ctx, throttle.wait(round));
+    if (ctx->take_cancel()) {
+      AUR_HIP(ctx, hipStreamSynchronize(s));
+      return ctx->fail(AUR_ERR_CANCELLED, "render cancelled");
+    }
+    c.s0 = sa;
+    for (uint32_t l0 = 0; l0 < npix; l0 += pix_cap) {
+      c.l0 = l0;
+      c.nchunk = std::min(pix_cap, npix - l0);
+      c.n = c.nchunk * ns;
+      k_cw_init<<<nblocks(c.n, 256), 256, 0, s>>>(c, a, fr.queue[0].p);
+      for (uint32_t it = 0;; ++it) {
+        const uint32_t w = it % 3u, wn = (it + 1u) % 3u;
+        trace(w, 1u, (it & 1u) * cap);
+        k_cw_shade<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, fr.queue[it & 1u].p);
+        if (nslots) trace(w, 2u, 0u);
+        k_cw_resolve<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, wn, fr.queue[it & 1u].p,
+                                                            fr.queue[(it + 1u) & 1u].p, ((it + 1u) & 1u) * cap);
+        AUR_LAUNCH_CHECK(ctx, "compat wavefront");
+        if (!ctx->child_shaders) break;  // no stack holds a second ray
+        uint32_t more = 0;
+        AUR_HIP(ctx, hipMemcpyAsync(&more, fr.counters.p + 2 * wn, sizeof(more), hipMemcpyDeviceToHost, s));
+        AUR_HIP(ctx, hipStreamSynchronize(s));
+        if (more == 0) break;
+        if (it + 1 >= kMaxLevels) return ctx->fail(AUR_ERR_DEVICE, "render: compat ray stack did not empty");
+      }
+      k_cw_accumulate<<<nblocks(c.nchunk, 256), 256, 0, s>>>(c, ns);
+      AUR_LAUNCH_CHECK(ctx, "compat accumulate");
+    }
+    AUR_HIP(ctx, throttle.mark(round));
+    ++round;
+  }
+  return AUR_OK;
+}
+
 // 3x3 tent filter of the region (cudaFilterPixelsKernel,
 // src/kernels/utility.cu:231-258): weights 1/16, 1/8, 1/4, taps clamped to
 // the region, sums in the reference's order.  The reference clamps y + 1 to
@@ -1356,37 +1652,14 @@ aur_status frame_render(aur_context* ctx, const aur_camera* cam, const aur_rende
   if (chk != AUR_OK) return chk;
   FrameState& fr = ctx->frame;
   hipStream_t s = ctx->stream;
-  const uint32_t w = fr.width, h = fr.height;
   const uint32_t rw = fr.right - fr.left + 1, rh = fr.top - fr.bottom + 1, npix = rw * rh;
   AUR_HIP(ctx, fr.counters.reserve(kCounterWords));
   unsigned long long* stats = reinterpret_cast<unsigned long long*>(fr.counters.p + kStatsOffset);
   k_zero_u64<<<1, 64, 0, s>>>(stats, 2);
   if (o->mode == AUR_MODE_COMPAT) {
-    SceneDev sc{ctx->nodes.p,        ctx->count / 2,    ctx->rec_bytes,   ctx->normals.p,
-                ctx->shader_ids.p,   ctx->shaders.p,    ctx->num_shaders, ctx->lights.p,
-                ctx->num_lights,     ctx->texcoords.p,  ctx->textures.p,  ctx->texels.p,
-                ctx->num_textures};
-    const size_t lds = (size_t)ctx->levels * kBlock * sizeof(float2);
-    const uint4 region = make_uint4(fr.left, fr.bottom, rw, npix);
-    // waiting mirror rays exist only under shaders that are both reflective
-    // and translucent
-    const uint32_t lanes = nblocks(npix, kBlock) * kBlock;
-    float4* pend = nullptr;
-    if (ctx->branching_shaders) {
-      AUR_HIP(ctx, fr.compat_pend.reserve((size_t)kCompatPend * 2 * lanes));
-      pend = fr.compat_pend.p;
-    }
-    PassThrottle throttle(s);
-    for (uint32_t smp = s0; smp < s1; ++smp) {
-      AUR_HIP(ctx, throttle.wait(smp - s0));
-      if (ctx->take_cancel()) {
-        AUR_HIP(ctx, hipStreamSynchronize(s));
-        return ctx->fail(AUR_ERR_CANCELLED, "render cancelled");
-      }
-      k_compat<<<nblocks(npix, kBlock), kBlock, lds, s>>>(sc, *cam, w, h, region, o->spp, smp, o->seed,
-                                                          fr.pixels.p, stats, pend, lanes);
-      AUR_LAUNCH_CHECK(ctx, "k_compat");
-      AUR_HIP(ctx, throttle.mark(smp - s0));
+    if (s1 > s0) {
+      const aur_status st = compat_passes(ctx, cam, o, s0, s1, stats);
+      if (st != AUR_OK) return st;
     }
   } else if (s1 > s0) {
     const JobDev J = frame_job(ctx, cam, o);
