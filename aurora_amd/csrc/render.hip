@@ -920,16 +920,23 @@ __global__ void k_cw_init(CwArgs c, PbrArgs a, uint32_t* __restrict__ qin) {
   qin[l] = l;
 }
 
-// One ray of each active pixel's stack, up to the shadow rays.
-__global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32_t w,
-                                                     const uint32_t* __restrict__ qin) {
+// One ray of each active entry's stack, up to the shadow rays; the entries
+// whose stacks go on enter the next level's queue with their next ray.
+// Queue counts as in the PBR passes: word w holds this level's entries;
+// shadow rays and the next level's entries are appended to word wn.
+__global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32_t w, uint32_t wn,
+                                                     const uint32_t* __restrict__ qin,
+                                                     uint32_t* __restrict__ qout, uint32_t roff_next) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = a.counters[2 * w];
   const bool act = i < n;
   const int kMaxDepth = 8;
   uint32_t ncast = 0;
+  bool more = false;
+  uint32_t l = 0;
+  CRay next{};
   if (act) {
-    const uint32_t l = qin[i];
+    l = qin[i];
     const float4 so = c.st_o[l], sd = c.st_d[l], sc4 = c.st_c[l];
     uint4 si = c.st_i[l];
     int depth = (int)si.x, np = (int)si.y;
@@ -1057,7 +1064,9 @@ __global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32
       c.emw[i] = make_float4(em.x, em.y, em.z, ray.weight);
       c1v = make_float4(color1.x, color1.y, color1.z, 0.0f);
     }
-    if (depth <= kMaxDepth && have) flags |= kCwMore;
+    more = depth <= kMaxDepth && have;
+    if (more) flags |= kCwMore;
+    next = top;
     c1v.w = __uint_as_float(flags);
     c.c1[i] = c1v;
     c.st_o[l] = make_float4(top.pos.x, top.pos.y, top.pos.z, top.weight);
@@ -1076,7 +1085,7 @@ __global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32
   for (uint32_t slot = 0; slot < c.nslots; ++slot)
     total += (uint32_t)__popcll(__ballot(act && ncast && c.contrib[(size_t)slot * c.stride + i].w != 0.0f));
   uint32_t run = 0;
-  if (lane == 0 && total) run = atomicAdd(&a.counters[2 * w + 1], total);
+  if (lane == 0 && total) run = atomicAdd(&a.counters[2 * wn + 1], total);
   run = __shfl(run, 0);
   for (uint32_t slot = 0; slot < c.nslots && total; ++slot) {
     const size_t e = (size_t)slot * c.stride + i;
@@ -1084,6 +1093,12 @@ __global__ __launch_bounds__(kBlock) void k_cw_shade(CwArgs c, PbrArgs a, uint32
     const uint64_t m = __ballot(cast);
     if (cast) a.sh_queue[run + (uint32_t)__popcll(m & lt)] = (uint32_t)e;
     run += (uint32_t)__popcll(m);
+  }
+  const uint32_t p = wave_append(&a.counters[2 * wn], more ? 1u : 0u);
+  if (more) {
+    qout[p] = l;
+    a.ray_o[roff_next + p] = make_float4(next.pos.x, next.pos.y, next.pos.z, 0.0f);
+    a.ray_d[roff_next + p] = make_float4(next.dir.x, next.dir.y, next.dir.z, 0.0f);
   }
 }
 
@@ -1111,16 +1126,12 @@ __global__ void k_cw_accumulate(CwArgs c, uint32_t ns) {
   c.pixels[pix] = px;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cw_resolve(CwArgs c, PbrArgs a, uint32_t w, uint32_t wn,
-                                                       const uint32_t* __restrict__ qin,
-                                                       uint32_t* __restrict__ qout, uint32_t roff_next) {
+__global__ __launch_bounds__(kBlock) void k_cw_resolve(CwArgs c, PbrArgs a, uint32_t w,
+                                                       const uint32_t* __restrict__ qin) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = a.counters[2 * w];
-  const bool act = i < n;
-  bool more = false;
-  uint32_t l = 0;
-  if (act) {
-    l = qin[i];
+  if (i < n) {
+    const uint32_t l = qin[i];
     const float4 c1 = c.c1[i];
     const uint32_t flags = __float_as_uint(c1.w);
     float4 sc4 = c.st_c[l];
@@ -1159,18 +1170,10 @@ __global__ __launch_bounds__(kBlock) void k_cw_resolve(CwArgs c, PbrArgs a, uint
       }
       color = color + (Li * emw.w);
     }
-    more = (flags & kCwMore) != 0u;
-    if (more)
+    if (flags & kCwMore)
       c.st_c[l] = make_float4(color.x, color.y, color.z, sc4.w);
     else
       c.fin[l] = make_float4(color.x, color.y, color.z, __float_as_int(sc4.w) != -1 ? 1.0f : 0.0f);
-  }
-  const uint32_t p = wave_append(&a.counters[2 * wn], more ? 1u : 0u);
-  if (more) {
-    const float4 so = c.st_o[l], sd = c.st_d[l];
-    qout[p] = l;
-    a.ray_o[roff_next + p] = make_float4(so.x, so.y, so.z, 0.0f);
-    a.ray_d[roff_next + p] = make_float4(sd.x, sd.y, sd.z, 0.0f);
   }
 }
 
@@ -1555,19 +1558,28 @@ aur_status compat_passes(aur_context* ctx, const aur_camera* cam, const aur_rend
       c.nchunk = std::min(pix_cap, npix - l0);
       c.n = c.nchunk * ns;
       k_cw_init<<<nblocks(c.n, 256), 256, 0, s>>>(c, a, fr.queue[0].p);
+      // level it: one launch traces its rays with level it - 1's shadow
+      // rays, then level it - 1's sums, then its shading; after the last
+      // level, its shadow rays and sums
       for (uint32_t it = 0;; ++it) {
-        const uint32_t w = it % 3u, wn = (it + 1u) % 3u;
-        trace(w, 1u, (it & 1u) * cap);
-        k_cw_shade<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, fr.queue[it & 1u].p);
-        if (nslots) trace(w, 2u, 0u);
-        k_cw_resolve<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, wn, fr.queue[it & 1u].p,
-                                                            fr.queue[(it + 1u) & 1u].p, ((it + 1u) & 1u) * cap);
+        const uint32_t w = it % 3u, wn = (it + 1u) % 3u, wp = (it + 2u) % 3u;
+        trace(w, 1u | (it > 0 && nslots ? 2u : 0u), (it & 1u) * cap);
+        if (it > 0)
+          k_cw_resolve<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, wp, fr.queue[(it + 1u) & 1u].p);
+        k_cw_shade<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, wn, fr.queue[it & 1u].p,
+                                                          fr.queue[(it + 1u) & 1u].p, ((it + 1u) & 1u) * cap);
         AUR_LAUNCH_CHECK(ctx, "compat wavefront");
-        if (!ctx->child_shaders) break;  // no stack holds a second ray
         uint32_t more = 0;
-        AUR_HIP(ctx, hipMemcpyAsync(&more, fr.counters.p + 2 * wn, sizeof(more), hipMemcpyDeviceToHost, s));
-        AUR_HIP(ctx, hipStreamSynchronize(s));
-        if (more == 0) break;
+        if (ctx->child_shaders) {  // else no stack holds a second ray
+          AUR_HIP(ctx, hipMemcpyAsync(&more, fr.counters.p + 2 * wn, sizeof(more), hipMemcpyDeviceToHost, s));
+          AUR_HIP(ctx, hipStreamSynchronize(s));
+        }
+        if (more == 0) {
+          if (nslots) trace(wn, 2u, 0u);
+          k_cw_resolve<<<nblocks(c.n, kBlock), kBlock, 0, s>>>(c, a, w, fr.queue[it & 1u].p);
+          AUR_LAUNCH_CHECK(ctx, "compat wavefront");
+          break;
+        }
         if (it + 1 >= kMaxLevels) return ctx->fail(AUR_ERR_DEVICE, "render: compat ray stack did not empty");
       }
       k_cw_accumulate<<<nblocks(c.nchunk, 256), 256, 0, s>>>(c, ns);

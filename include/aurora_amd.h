@@ -232,9 +232,11 @@ aur_status aur_frame_render(aur_context* ctx, const aur_camera* camera,
  * 65,536-ray tiles one sample at a time, src/render/monteCarloRaytracer.cpp:13).
  * max_paths: paths in flight per wavefront pass; a frame pass traces
  * clamp(max_paths / pixels, 1, remaining) samples of every pixel, a batch
- * pass max_paths paths.  0 selects the default (2^24).
- * graph: 1 captures one pass (camera, bounce loop, accumulation) in a HIP
- * graph and replays it for every pass; 0 launches the kernels directly.
+ * pass max_paths paths, a compat-mode round as many whole samples of every
+ * pixel (or pixels of one sample) as fit max_paths entries.  0 selects the
+ * default (2^24).
+ * graph: 1 captures one PBR pass (camera, bounce loop, accumulation) in a
+ * HIP graph and replays it for every pass; 0 launches the kernels directly.
  * refill: the render's persistent traversal launches refill idle lanes once
  * this many (1..64) are idle; 0 selects the default (16).
  * stack_lds_rows: how many rows of each lane's traversal stack live in LDS
