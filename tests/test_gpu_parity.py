@@ -228,6 +228,30 @@ def test_render_progressive_equals_full(gpu_ctx):
     assert np.array_equal(full, split)
 
 
+@pytest.mark.parametrize("max_paths,refill,rows", [(64 * 64, 0, 64), (3 * 64 * 64, 1, 0), (1000, 16, 8),
+                                                   (0, 40, 3)])
+def test_render_compat_rounds_invariant(gpu_ctx, max_paths, refill, rows):
+    """Compat rounds: (pixel, sample) entries per round (one sample, three
+    samples, chunks of 1000 pixels, all eight samples), the traversal refill
+    threshold and the LDS window of its stack must not change a single bit,
+    on a scene whose mirror materials take several stack levels."""
+    s = scenes.spheres(n_spheres=30, width=64, height=64)
+    _build(gpu_ctx, s)
+    gpu_ctx.frame_create(64, 64, 8)
+    o = native.Context.opts(native.MODE_COMPAT, 8, 8, 3, 5)
+    try:
+        gpu_ctx.set_render_params(0)  # all eight samples in one round
+        gpu_ctx.frame_render(s.camera, o)
+        ref = gpu_ctx.frame_read()
+        gpu_ctx.set_render_params(max_paths, 0, refill, rows)
+        gpu_ctx.frame_render(s.camera, o)
+        img = gpu_ctx.frame_read()
+    finally:
+        gpu_ctx.set_render_params(0)
+    assert ref[:, 3].max() > 0
+    assert np.array_equal(ref, img)
+
+
 @pytest.mark.parametrize("max_paths,refill,rows", [(1, 0, 64), (3 * 64 * 64, 1, 0), (0, 0, 0),
                                                    (0, 40, 64), (0, 0, 3), (5000, 16, 8)])
 def test_render_pass_batching_invariant(gpu_ctx, max_paths, refill, rows):
