@@ -1470,8 +1470,9 @@ aur_status compat_passes(aur_context* ctx, const aur_camera* cam, const aur_rend
   const uint32_t rw = fr.right - fr.left + 1, npix = rw * (fr.top - fr.bottom + 1);
   const uint32_t nslots = ctx->compat_slots;
   const uint64_t per_pixel = 1ull + nslots;
+  // entries per round: max_paths, and at most 2^26 shadow slots (72 B each)
   const uint64_t budget = std::min<uint64_t>(std::max<uint32_t>(ctx->render_params.max_paths, 1u),
-                                             0xFFFFFFFFull / std::max<uint32_t>(nslots, 1u));
+                                             std::max<uint64_t>(1u, (1ull << 26) / std::max<uint32_t>(nslots, 1u)));
   const uint32_t pix_cap = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(npix, budget));
   const uint32_t ns_max = pix_cap == npix ? (uint32_t)std::max<uint64_t>(1u, budget / npix) : 1u;
   const uint32_t cap = pix_cap * ns_max;  // entries per round
