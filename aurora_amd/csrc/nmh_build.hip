@@ -17,10 +17,15 @@
 //     lexicographic comparison of the later levels' keys (newest first; only
 //     the last three levels can differ since axes cycle with period 3);
 //   * while node groups are larger than one workgroup's LDS tile, a level is a
-//     segmented stable LSD radix sort (4 x 8-bit digits, tile histograms,
+//     segmented stable LSD radix sort (8-bit digits, tile histograms,
 //     per-group digit offsets, wave-ballot stable ranking) followed by a
 //     tiled first-maximum reduction and an emit pass that writes finished
-//     pairs and compacts the remainders for the next level;
+//     pairs and compacts the remainders for the next level.  The digits are
+//     those of key - (the group's smallest key), so a group whose keys span
+//     fewer than 2^24 values needs 3 passes (2^16: 2, ...): the emit pass
+//     records each tile's key range, the next level's first group scan
+//     reduces them per group and hands (smallest key, passes) to every tile
+//     and chunk of the group;
 //   * once every group fits (<= kSubT elements) one workgroup takes a whole
 //     subtree and runs all remaining levels in LDS with a bitonic sort on
 //     (group, key, position) composite keys.
@@ -114,6 +119,21 @@ __global__ void k_iota(uint2* __restrict__ V, uint32_t N, const float2* __restri
 // tests, never as an out-of-bounds access.
 __device__ __forceinline__ uint32_t clampi(uint32_t i, uint32_t N) { return i < N ? i : N - 1; }
 
+// Radix passes a group needs: its keys minus the smallest, k - r.x, span
+// [0, r.y - r.x]; digits above the span's top byte are zero in every key, and
+// a stable sort by a digit that is equal for all keys keeps the order.  Pass
+// 0 runs for every group (its digit, (k - r.x) & 255, is a rotation of the
+// key's low byte, and the group's range is known only after the pass's
+// histograms).  Pass p of a level moves the groups with group_passes > p,
+// from buffer p & 1 to the other, so a group's sorted data ends in buffer
+// group_passes & 1.  Kernels read a tile's (smallest key, passes) as
+// TP[tile] and a chunk's passes as CP[chunk] (by block index, so the load
+// does not wait for the tile's descriptor).
+__device__ __forceinline__ uint32_t group_passes(uint2 r) {
+  const uint32_t d = r.y - r.x;
+  return d < 0x10000u ? (d < 0x100u ? 1u : 2u) : (d < 0x1000000u ? 3u : 4u);
+}
+
 __global__ void k_fill0(uint32_t* __restrict__ p, uint32_t N) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N) p[i] = 0;
@@ -122,19 +142,28 @@ __global__ void k_fill0(uint32_t* __restrict__ p, uint32_t N) {
 // Per-tile digit histogram H[tile][digit], also added into its chunk's digit
 // counts CS[chunk][digit] (integer atomics: the sums are exact whatever the
 // order).  CS starts at zero: zeroed once per build and again by
-// k_radix_group_scan after it reads a pass's counts.
+// k_radix_group_scan after it reads a pass's counts.  Pass 0 counts the
+// key's low byte (indexed so; k_radix_group_scan orders the digits).
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restrict__ tiles,
                                                          const GroupDesc* __restrict__ groups,
+                                                         const uint2* __restrict__ TP,
                                                          const uint32_t* __restrict__ K,
-                                                         uint32_t shift,
+                                                         uint32_t pass,
                                                          uint32_t* __restrict__ H,
                                                          uint32_t* __restrict__ CS) {
   __shared__ uint32_t h[256];
   const TileDesc t = tiles[blockIdx.x];
+  uint32_t kmin = 0;
+  if (pass > 0) {
+    const uint2 tp = TP[blockIdx.x];
+    if (tp.y <= pass) return;  // uniform per block
+    kmin = tp.x;
+  }
+  const uint32_t shift = 8u * pass;
   h[threadIdx.x] = 0;
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < t.count; i += kThreads)
-    atomicAdd(&h[(K[t.start + i] >> shift) & 255u], 1u);
+    atomicAdd(&h[((K[t.start + i] - kmin) >> shift) & 255u], 1u);
   __syncthreads();
   const uint32_t v = h[threadIdx.x];
   H[(size_t)blockIdx.x * 256 + threadIdx.x] = v;
@@ -146,16 +175,64 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const TileDesc* __restr
 // Per group: exclusive digit offsets of the group (digit-major), and the base
 // B[chunk][digit] of every chunk of the group (the counts of the group's
 // earlier chunks added); then the chunk counts are zeroed for the next pass.
+// (A group that skips the pass has no counts: they stay zero.)
+// Pass 0 first sets the group's key range: from the previous level's emit
+// tiles of the group's parent node (prev_children[group], tile ranges TR),
+// or the full range on the first level (prev_children null); GP[group],
+// TP[tile] and CP[chunk] of the group take (smallest key, passes).  Its
+// digits are the keys' low bytes, scanned in the order of
+// (low byte - smallest key) & 255.
 __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* __restrict__ groups,
+                                                               uint2* __restrict__ GP,
+                                                               uint2* __restrict__ TP,
+                                                               uint32_t* __restrict__ CP,
+                                                               const ChildDesc* __restrict__ prev_children,
+                                                               const uint2* __restrict__ TR,
+                                                               uint32_t pass,
                                                                uint32_t* __restrict__ S,
                                                                uint32_t* __restrict__ B) {
   __shared__ uint32_t s[256];
-  const GroupDesc g = groups[blockIdx.x];
+  __shared__ uint32_t rmin[kThreads / 64], rmax[kThreads / 64];
   const uint32_t d = threadIdx.x;
+  uint32_t rot = 0;  // scan position of digit d: (d - rot) & 255
+  const GroupDesc g = groups[blockIdx.x];
+  const bool reduce = pass == 0 && prev_children != nullptr;
+  const ChildDesc pc = reduce ? prev_children[blockIdx.x] : ChildDesc{0u, 0u, 0u, 0u};
+  if (pass > 0 && GP[blockIdx.x].y <= pass) return;  // uniform per block
+  // the group's digit counts first: their loads then overlap the range's
   uint32_t total = 0;
 #pragma unroll 8
   for (uint32_t c = g.cb; c < g.ce; ++c) total += S[(size_t)c * 256 + d];
-  s[d] = total;
+  if (pass == 0) {
+    uint2 r = make_uint2(0u, 0xFFFFFFFFu);
+    if (reduce) {
+      uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+      for (uint32_t t = pc.tb + d; t < pc.te; t += kThreads) {
+        const uint2 tr = TR[t];
+        mn = min(mn, tr.x);
+        mx = max(mx, tr.y);
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, off));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+      }
+      if ((d & 63u) == 0) {
+        rmin[d >> 6] = mn;
+        rmax[d >> 6] = mx;
+      }
+      __syncthreads();
+      r = make_uint2(rmin[0], rmax[0]);
+#pragma unroll
+      for (int w = 1; w < kThreads / 64; ++w) r = make_uint2(min(r.x, rmin[w]), max(r.y, rmax[w]));
+    }
+    const uint2 gp = make_uint2(r.x, group_passes(r));
+    if (d == 0) GP[blockIdx.x] = gp;
+    for (uint32_t t = g.tb + d; t < g.te; t += kThreads) TP[t] = gp;
+    for (uint32_t c = g.cb + d; c < g.ce; c += kThreads) CP[c] = gp.y;
+    rot = r.x & 255u;
+  }
+  const uint32_t sp = (d - rot) & 255u;
+  s[sp] = total;
   __syncthreads();
   for (uint32_t off = 1; off < 256; off <<= 1) {
     const uint32_t v = d >= off ? s[d - off] : 0u;
@@ -163,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
     s[d] += v;
     __syncthreads();
   }
-  uint32_t run = g.off + s[d] - total;
+  uint32_t run = g.off + s[sp] - total;
 #pragma unroll 8
   for (uint32_t c = g.cb; c < g.ce; ++c) {
     B[(size_t)c * 256 + d] = run;
@@ -176,10 +253,13 @@ __global__ __launch_bounds__(kThreads) void k_radix_group_scan(const GroupDesc* 
 // counts of the chunk's earlier tiles (one block per chunk; the chunk's
 // tiles are found from the tile's group).
 __global__ __launch_bounds__(kThreads) void k_radix_chunk_offsets(const uint32_t* __restrict__ chunk_tile,
+                                                                  const uint32_t* __restrict__ CP,
+                                                                  uint32_t pass,
                                                                   const uint32_t* __restrict__ H,
                                                                   const uint32_t* __restrict__ B,
                                                                   uint32_t* __restrict__ O) {
   const uint32_t tb = chunk_tile[blockIdx.x], te = chunk_tile[blockIdx.x + 1];
+  if (pass > 0 && CP[blockIdx.x] <= pass) return;  // uniform per block
   const uint32_t d = threadIdx.x;
   uint32_t run = B[(size_t)blockIdx.x * 256 + d];
 #pragma unroll 8
@@ -217,9 +297,10 @@ __device__ __forceinline__ uint32_t block_incl_scan256(uint32_t v, uint32_t* __r
 // a 16-bit tile index and re-reading the payload at write-out measured 6%
 // slower despite the smaller LDS footprint.)
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(
-    const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ Kin,
-    const uint2* __restrict__ Vin, uint32_t shift, const uint32_t* __restrict__ O, uint32_t m,
-    uint32_t* __restrict__ Kout, uint2* __restrict__ Vout) {
+    const TileDesc* __restrict__ tiles, const uint2* __restrict__ TP,
+    const uint32_t* __restrict__ Kin, const uint2* __restrict__ Vin, uint32_t pass,
+    const uint32_t* __restrict__ O, uint32_t m, uint32_t* __restrict__ Kout,
+    uint2* __restrict__ Vout) {
   constexpr uint32_t kWaves = kThreads / 64;
   constexpr uint32_t kRounds = kTile / kThreads;  // rounds of 64 per wave
   __shared__ uint32_t sk[kTile];
@@ -228,9 +309,16 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   __shared__ uint32_t delta[256];       // output position - tile position, per digit
   __shared__ uint32_t wsum[kWaves];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const TileDesc t = tiles[blockIdx.x];
+  uint32_t kmin = 0;  // pass 0: the low byte itself (the offsets O follow the group's digit order)
+  if (pass > 0) {
+    const uint2 tp = TP[blockIdx.x];
+    if (tp.y <= pass) return;  // uniform per block
+    kmin = tp.x;
+  }
+  const uint32_t shift = 8u * pass;
 #pragma unroll
   for (uint32_t q = 0; q < 4; ++q) wc[wave][q * 64 + lane] = 0;
-  const TileDesc t = tiles[blockIdx.x];
   const uint32_t w0 = wave * (kRounds * 64);
   uint32_t k[kRounds], loc[kRounds];
   uint2 v[kRounds];
@@ -245,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 #pragma unroll
   for (uint32_t r = 0; r < kRounds; ++r) {
     const bool valid = w0 + r * 64 + lane < t.count;
-    const uint32_t dg = (k[r] >> shift) & 255u;
+    const uint32_t dg = ((k[r] - kmin) >> shift) & 255u;
     uint64_t mask = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -281,7 +369,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 #pragma unroll
   for (uint32_t r = 0; r < kRounds; ++r) {
     if (w0 + r * 64 + lane < t.count) {
-      const uint32_t pos = wc[wave][(k[r] >> shift) & 255u] + loc[r];
+      const uint32_t pos = wc[wave][((k[r] - kmin) >> shift) & 255u] + loc[r];
       if (pos < kTile) {
         sk[pos] = k[r];
         sv[pos] = v[r];
@@ -292,7 +380,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   const uint32_t n_out = min(t.count, kTile);
   for (uint32_t j = tid; j < n_out; j += kThreads) {
     const uint32_t key = sk[j];
-    const uint32_t pos = delta[(key >> shift) & 255u] + j;
+    const uint32_t pos = delta[((key - kmin) >> shift) & 255u] + j;
     if (pos < m) {
       Kout[pos] = key;
       Vout[pos] = sv[j];
@@ -301,14 +389,18 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
 }
 
 // First maximum of the max key over child positions [off+1, off+size)
-// restricted to this tile: (key, position), key 0 = none.
+// restricted to this tile: (key, position), key 0 = none.  The group's sorted
+// data is in V0 or V1 (group_passes).
 __global__ __launch_bounds__(kThreads) void k_argmax_partial(const TileDesc* __restrict__ tiles,
                                                              const ChildDesc* __restrict__ children,
-                                                             const uint2* __restrict__ V,
+                                                             const uint2* __restrict__ TP,
+                                                             const uint2* __restrict__ V0,
+                                                             const uint2* __restrict__ V1,
                                                              uint2* __restrict__ partial) {
   __shared__ uint32_t sk[kThreads / 64], sp[kThreads / 64];
   const TileDesc t = tiles[blockIdx.x];
   const ChildDesc c = children[t.child];
+  const uint2* __restrict__ V = (TP[blockIdx.x].y & 1u) ? V1 : V0;
   const uint32_t lo = max(t.start, c.off + 1), hi = t.start + t.count;
   uint32_t bk = 0, bp = 0xFFFFFFFFu;
   for (uint32_t i = lo + threadIdx.x; i < hi; i += kThreads) {
@@ -368,28 +460,37 @@ __global__ __launch_bounds__(kThreads) void k_argmax_combine(const ChildDesc* __
   if (lane == 0) argpos[c] = bp;
 }
 
-// Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders;
-// with Knext, also the next level's sort keys and max keys of the compacted
-// remainders (one 8-byte load of their bounds on the next axis).  With orig
-// (the last level before the subtree phase), the remainders are renumbered
-// instead: position dst takes the place of triangle id val, orig[dst] = val,
-// and its bounds on all three axes go to bbR[axis][dst] (mR per axis), so
-// that every subtree group reads one contiguous window of bounds.
+// Writes finished pairs (tree positions 2h, 2h+1) and compacts remainders
+// (read from V0 or V1 as the group's sort left them, written to Vnext); with
+// Knext, also the next level's sort keys and max keys of the compacted
+// remainders (one 8-byte load of their bounds on the next axis), and the
+// tile's (smallest, largest) key in TR[tile] ((~0, 0) if it has none).  A
+// tile's remainders all go to the next level's group of the tile's node.
+// With orig (the last level before the subtree phase),
+// the remainders are renumbered instead: position dst takes the place of
+// triangle id val, orig[dst] = val, and its bounds on all three axes go to
+// bbR[axis][dst] (mR per axis), so that every subtree group reads one
+// contiguous window of bounds.
 __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ tiles,
                                                    const ChildDesc* __restrict__ children,
                                                    const uint32_t* __restrict__ argpos,
-                                                   const uint2* __restrict__ V,
+                                                   const uint2* __restrict__ TP,
+                                                   const uint2* __restrict__ V0,
+                                                   const uint2* __restrict__ V1,
                                                    uint2* __restrict__ Vnext, uint32_t first,
                                                    int level, int L, const float2* __restrict__ bb,
                                                    uint32_t N, uint32_t* __restrict__ P,
                                                    uint32_t* __restrict__ Knext,
+                                                   uint2* __restrict__ TR,
                                                    float2* __restrict__ bbR, uint32_t* __restrict__ orig,
                                                    uint32_t mR) {
   const TileDesc t = tiles[blockIdx.x];
   const ChildDesc c = children[t.child];
+  const uint2* __restrict__ V = (TP[blockIdx.x].y & 1u) ? V1 : V0;
   const float2* bnext = bb + (size_t)((level + 1) % 3) * N;
   uint32_t am = argpos[t.child];
   if (am <= c.off || am >= c.off + c.size) am = c.off + 1;  // cannot happen for sizes >= 2
+  uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
   for (uint32_t r = threadIdx.x; r < t.count; r += kThreads) {
     const uint32_t i = t.start + r;
     if (i == c.off) {
@@ -399,8 +500,11 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
       const uint32_t dst = i - 2 * (t.child + 1);
       if (Knext) {
         const float2 b = bnext[val];
-        Knext[dst] = fkey(b.x);
+        const uint32_t key = fkey(b.x);
+        Knext[dst] = key;
         Vnext[dst] = make_uint2(val, fkey(b.y));
+        kmin = min(kmin, key);
+        kmax = max(kmax, key);
       } else if (orig) {
         const float2 b0 = bb[val], b1 = bb[(size_t)N + val], b2 = bb[2 * (size_t)N + val];
         if (dst < mR) {
@@ -413,6 +517,26 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TileDesc* __restrict__ 
       } else {
         Vnext[dst] = make_uint2(val, 0u);
       }
+    }
+  }
+  if (Knext) {  // uniform: a kernel argument
+    __shared__ uint32_t rmin[kThreads / 64], rmax[kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+      kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
+      kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
+    }
+    if ((threadIdx.x & 63u) == 0) {
+      rmin[threadIdx.x >> 6] = kmin;
+      rmax[threadIdx.x >> 6] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int w = 1; w < kThreads / 64; ++w) {
+        kmin = min(kmin, rmin[w]);
+        kmax = max(kmax, rmax[w]);
+      }
+      TR[blockIdx.x] = make_uint2(kmin, kmax);
     }
   }
 }
@@ -875,7 +999,11 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   // ---- device buffers ----
   DevBuf<float2> bb;          // (min, max) per axis and triangle
   DevBuf<float2> bbR;         // the same for the subtree phase's pending region, by position
-  DevBuf<uint2> Va, Vb;        // sort payload: (triangle id, max key on the level's axis)
+  // sort payload (triangle id, max key on the level's axis): the level's
+  // input, the other side of the radix passes, and the emit pass's output
+  DevBuf<uint2> Va, Vb, Vc;
+  DevBuf<uint2> GP, TP, TR;  // (smallest key, passes) per group and tile; emit tiles' key ranges
+  DevBuf<uint32_t> CP;        // passes per offset chunk
   DevBuf<uint32_t> Ka, Kb, H, O, CS, CB, argpos, dchunk_tiles;
   DevBuf<uint2> partial;
   DevBuf<TileDesc> dtiles;
@@ -887,6 +1015,11 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   AUR_HIP(ctx, Vb.reserve(N));
   AUR_HIP(ctx, ctx->perm.reserve(N));
   if (!plan.empty()) {
+    AUR_HIP(ctx, Vc.reserve(N));
+    AUR_HIP(ctx, GP.reserve(max_children));
+    AUR_HIP(ctx, TP.reserve(max_tiles));
+    AUR_HIP(ctx, CP.reserve(max_chunks));
+    AUR_HIP(ctx, TR.reserve(max_tiles));
     AUR_HIP(ctx, Ka.reserve(N));
     AUR_HIP(ctx, Kb.reserve(N));
     AUR_HIP(ctx, H.reserve(max_tiles * 256));
@@ -921,36 +1054,45 @@ aur_status nmh_build(aur_context* ctx, aur_build_stats* stats) {
   k_bounds<<<blocks(N, 256), 256, 0, s>>>(ctx->in_vertices.p, N, bb.p);
   // level 0's keys come with the identity order; every later level's keys
   // and max keys are gathered by the previous level's emit pass
-  k_iota<<<blocks(N, 256), 256, 0, s>>>(Va.p, N, bb.p, plan.empty() ? nullptr : Ka.p);
+  uint2* Vcur = Va.p;  // the level's input (k_iota's, then the previous emit's output)
+  uint2* Valt = Vb.p;  // the other side of the radix passes
+  uint2* Vout = Vc.p;  // the emit pass's output
+  k_iota<<<blocks(N, 256), 256, 0, s>>>(Vcur, N, bb.p, plan.empty() ? nullptr : Ka.p);
   k_fill0<<<blocks(N, 256), 256, 0, s>>>(ctx->perm.p, N);
   if (!plan.empty()) k_fill0<<<blocks(max_chunks * 256, 256), 256, 0, s>>>(CS.p, (uint32_t)(max_chunks * 256));
-  uint2* Vcur = Va.p;
-  uint2* Valt = Vb.p;
   for (size_t li = 0; li < plan.size(); ++li) {
     const LevelPlan& lp = plan[li];
     const TileDesc* tl = dtiles.p + lp.tile_base;
     const ChildDesc* ch = dchildren.p + lp.child_base;
     const GroupDesc* gr = dgroups.p + lp.group_base;
+    // the previous level's nodes: group g of this level is node g's remainder
+    const ChildDesc* prev = li == 0 ? nullptr : dchildren.p + plan[li - 1].child_base;
     uint32_t *kin = Ka.p, *kout = Kb.p;
     uint2 *vin = Vcur, *vout = Valt;
-    for (int pass = 0; pass < 4; ++pass) {
-      const uint32_t shift = 8u * pass;
-      k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, gr, kin, shift, H.p, CS.p);
-      k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, CS.p, CB.p);
-      k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(dchunk_tiles.p + lp.chunk_base, H.p, CB.p, O.p);
-      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, kin, vin, shift, O.p, lp.m, kout, vout);
+    // every launch of a pass skips the groups that need fewer passes
+    for (uint32_t pass = 0; pass < 4; ++pass) {
+      k_radix_hist<<<lp.ntiles, kThreads, 0, s>>>(tl, gr, TP.p, kin, pass, H.p, CS.p);
+      k_radix_group_scan<<<lp.ngroups, kThreads, 0, s>>>(gr, GP.p, TP.p, CP.p, prev, TR.p, pass, CS.p, CB.p);
+      k_radix_chunk_offsets<<<lp.nchunks, kThreads, 0, s>>>(dchunk_tiles.p + lp.chunk_base, CP.p, pass,
+                                                            H.p, CB.p, O.p);
+      k_radix_scatter<<<lp.ntiles, kThreads, 0, s>>>(tl, TP.p, kin, vin, pass, O.p, lp.m, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);
     }
-    // after 4 passes the sorted data is back in (Ka, Vcur); Ka is free again
-    k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, Vcur, partial.p);
+    // each group's sorted data is in Vcur or Valt (group_passes); the keys
+    // are not read again, so Ka takes the next level's
+    const bool last = li + 1 == plan.size();
+    k_argmax_partial<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, TP.p, Vcur, Valt, partial.p);
     k_argmax_combine<<<blocks(lp.nchildren, kThreads / 64), kThreads, 0, s>>>(ch, lp.nchildren,
                                                                               partial.p, argpos.p);
-    const bool last = li + 1 == plan.size();
-    k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, Vcur, Valt, lp.first, lp.level, L,
-                                          bb.p, N, ctx->perm.p, last ? nullptr : Ka.p,
+    k_emit<<<lp.ntiles, kThreads, 0, s>>>(tl, ch, argpos.p, TP.p, Vcur, Valt, Vout, lp.first, lp.level, L,
+                                          bb.p, N, ctx->perm.p, last ? nullptr : Ka.p, TR.p,
                                           last ? bbR.p : nullptr, last ? Ka.p : nullptr, m0);
-    std::swap(Vcur, Valt);
+    // rotate: the emit output is the next level's input
+    uint2* const free_buf = Vcur;
+    Vcur = Vout;
+    Vout = Valt;
+    Valt = free_buf;
   }
   AUR_LAUNCH_CHECK(ctx, "nmh level kernels");
   if (level0 > 0) {

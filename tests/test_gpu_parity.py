@@ -37,6 +37,23 @@ def test_nmh_bitexact_random_soup(gpu_ctx, oracle, n, grid):
     assert np.array_equal(perm, ref)
 
 
+@pytest.mark.parametrize("span_log2", [-21, -16, -9])
+@pytest.mark.parametrize("n", [33000, 70000])
+def test_nmh_bitexact_narrow_keys(gpu_ctx, oracle, n, span_log2):
+    """Coordinates in [1, 1 + 2^span_log2): after the first level, the sort
+    keys of a group differ by fewer than 2^2, 2^7 or 2^14 float steps, so the
+    radix levels sort groups with one or two digit passes (and groups of
+    equal keys with none)."""
+    import dataclasses
+
+    s = scenes.random_soup(seed=n + span_log2, n=n)
+    rng = np.random.default_rng(n - span_log2)
+    v = (1.0 + rng.random((n, 9)) * 2.0 ** span_log2).astype(np.float32)
+    s = dataclasses.replace(s, vertices=scenes.canonical(v))
+    perm, _ = _build(gpu_ctx, s)
+    assert np.array_equal(perm, oracle.nmh_build(s.vertices))
+
+
 def test_nmh_bitexact_cornell(gpu_ctx, oracle):
     s = scenes.cornell()
     perm, _ = _build(gpu_ctx, s)
