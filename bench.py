@@ -266,11 +266,14 @@ def workload_soup10m(a) -> dict:
         popped, passed = ctx.trace_closest_stats_ptr(n, rays.data_ptr(), hits.data_ptr())
         model = 48 * n + 24 * popped + 48 * passed
         traffic = parse_traffic(f"k_trace_pt:{name}", "soup10m_traversal.md")
+        valu = parse_profile("valu_insts_per_launch", f"k_trace_pt:{name}", "soup10m_traversal.md")
         trace_roof[name] = {"ms": round(ms, 4), "popped_per_ray": round(popped / n, 3),
                             "passed_per_ray": round(passed / n, 3), "hbm_model_bytes": int(model),
                             "hbm_model_frac": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "traffic": traffic,
-                            "hbm_measured_frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None}
+                            "hbm_measured_frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                            "valu_insts_per_launch": valu,
+                            "valu_frac": round(valu * 64 / (ms * 1e-3) / 1e12 / VALU_PEAK_T, 4) if valu else None}
         log(f"{name}: P={popped / n:.1f} S={passed / n:.1f}, model {model / 1e9:.1f} GB per launch")
         del rays, hits
     cpu = None
