@@ -17,6 +17,9 @@ namespace aur {
 
 // PBR mode carries a vertex's next-event slots in one 64-bit mask.
 constexpr uint32_t kMaxSlots = 64;
+// Compat mode queues every shadow ray of a hit (a round holds at most 2^26
+// of them, so 4096 per hit still leaves 16,384 entries per round).
+constexpr uint32_t kMaxCompatSlots = 4096;
 
 // Next-event slots a light takes per path vertex in PBR mode: an area light
 // one per sample, an ambient light two (its unshadowed ambient part and its
@@ -27,7 +30,8 @@ __host__ __device__ inline uint32_t light_slots(const aur_light& L) {
 }
 
 // Slots of a light set, summed in 64 bits and saturated at 2^32 - 1: any sum
-// above kMaxSlots makes PBR renders refuse the set (compat mode has no limit).
+// above kMaxSlots makes PBR renders refuse the set (compat mode counts its
+// shadow rays per hit separately, up to kMaxCompatSlots).
 inline uint32_t light_set_slots(uint32_t count, const aur_light* lights) {
   uint64_t sum = 0;
   for (uint32_t i = 0; i < count; ++i) sum += light_slots(lights[i]);

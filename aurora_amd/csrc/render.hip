@@ -1654,6 +1654,8 @@ aur_status frame_render_check(aur_context* ctx, const aur_render_opts* o, uint32
   if (o->mode != AUR_MODE_COMPAT && o->mode != AUR_MODE_PBR) return ctx->fail(AUR_ERR_INVALID_ARG, "render: mode");
   if (o->mode == AUR_MODE_PBR && ctx->num_slots > kMaxSlots)
     return ctx->fail(AUR_ERR_UNSUPPORTED, "render: more than 64 next-event slots (light samples) per vertex");
+  if (o->mode == AUR_MODE_COMPAT && ctx->compat_slots > kMaxCompatSlots)
+    return ctx->fail(AUR_ERR_UNSUPPORTED, "render: more than 4096 compat shadow rays (light samples) per hit");
   const uint32_t npix = (fr.right - fr.left + 1) * (fr.top - fr.bottom + 1);
   if ((uint64_t)npix * o->spp > 0xFFFFFFFFull) return ctx->fail(AUR_ERR_INVALID_ARG, "render: pixels * spp exceeds 2^32");
   return AUR_OK;

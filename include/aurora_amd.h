@@ -61,7 +61,10 @@ typedef struct aur_shader {
  * direction = the way the light travels.  Ambient: the fraction `ambient`
  * lights every point unshadowed, the rest comes from position with a shadow
  * ray and no falloff.  Area: corner position, edges e1 and e2, emitting
- * along direction, area = |e1| |e2|, `samples` shadow rays per hit. */
+ * along direction, area = |e1| |e2|, `samples` shadow rays per hit.
+ * Renders refuse (AUR_ERR_UNSUPPORTED) light sets with more than 64 shadow
+ * rays per path vertex in PBR mode (an ambient light takes two) or more than
+ * 4096 per hit in compat mode. */
 enum { AUR_LIGHT_AMBIENT = 0, AUR_LIGHT_POINT = 1, AUR_LIGHT_DIRECTIONAL = 2, AUR_LIGHT_AREA = 3 };
 typedef struct aur_light {
   int32_t type;

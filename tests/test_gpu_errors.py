@@ -235,10 +235,32 @@ def test_next_event_slot_limit(fresh):
     assert float(out.abs().sum()) == 0.0
 
 
+def test_compat_shadow_ray_limit(fresh):
+    """Compat mode queues every shadow ray of a hit: an area light of 4096
+    samples renders (a round then holds 16,384 entries), 4097 are refused as
+    unsupported before any kernel runs, and the context keeps working."""
+    s = scenes.cornell(width=8, height=8, subdivisions=1)
+    fresh.upload_scene(s)
+    fresh.rebuild_nmh()
+    fresh.frame_create(8, 8, 1)
+    L = scenes.area_light((2.0, 4.9, 2.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, -1.0, 0.0), (15.0, 15.0, 15.0),
+                          samples=4097)
+    compat = native.Context.opts(native.MODE_COMPAT, 1)
+    fresh.upload_lights(np.array([L], dtype=scenes.LIGHT_DTYPE))
+    assert _status(fresh.frame_render, s.camera, compat) == UNSUPPORTED
+    assert float(np.abs(fresh.frame_read()).sum()) == 0.0
+    L["samples"] = 4096
+    fresh.upload_lights(np.array([L], dtype=scenes.LIGHT_DTYPE))
+    fresh.frame_render(s.camera, compat)
+    img = fresh.frame_read()
+    assert np.isfinite(img).all() and img[:, :3].max() > 0
+
+
 def test_slot_sum_does_not_wrap(fresh):
     """131,076 area lights of 32,767 samples and one of 68 take 2^32 + 64
     next-event slots: the count saturates instead of wrapping to 64, so PBR
-    renders and batches refuse the set before any kernel runs."""
+    renders, compat renders and batches refuse the set before any kernel
+    runs."""
     import torch
 
     s = scenes.cornell(width=8, height=8, subdivisions=1)
@@ -253,6 +275,7 @@ def test_slot_sum_does_not_wrap(fresh):
     fresh.upload_lights(lights)
     pbr = native.Context.opts(native.MODE_PBR, 1)
     assert _status(fresh.frame_render, s.camera, pbr) == UNSUPPORTED
+    assert _status(fresh.frame_render, s.camera, native.Context.opts(native.MODE_COMPAT, 1)) == UNSUPPORTED
     assert float(np.abs(fresh.frame_read()).sum()) == 0.0
     s.lights = lights
     sid = fresh.scene_add(s)
