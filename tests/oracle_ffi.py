@@ -33,13 +33,25 @@ def use_native() -> str:
 
 
 def cpu_model() -> str:
+    """The host CPU's model name, with its family/model numbers and the
+    number of logical CPUs from /proc/cpuinfo (a virtual machine may report
+    only a short name)."""
+    info: dict[str, str] = {}
+    ncpu = 0
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "processor":
+                ncpu += 1
+            elif k in ("model name", "cpu family", "model") and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info and "model" in info:
+        name += f" (family {info['cpu family']}, model {info['model']})"
+    return f"{name}, {ncpu} logical CPUs" if ncpu else name
 
 
 def lib(path: str | None = None):
