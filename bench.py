@@ -27,8 +27,9 @@ Other workloads print their own JSON line:
   soup10m          10M-triangle soup: NMH build ms, incoherent and coherent Mrays/s
   render_cornell   PBR seconds to 1% rel-RMSE, Cornell box 256^2
   render_spheres   PBR seconds to 1% rel-RMSE, 1M triangles 1080p
-  compat_cornell   compat-mode (reference MC ray tracer) frame, 256^2 16 spp
-  compat_spheres   compat-mode frame, 1M triangles, 1080p 64 spp
+  compat_cornell   compat-mode (reference MC ray tracer) frame, 256^2 16 spp, and
+                   the compat estimator's seconds to 1% rel-RMSE
+  compat_spheres   compat-mode frame, 1M triangles, 1080p 64 spp, and seconds to 1%
   batch256         256 batched jobs vs per-job frames
   adjoint_cornell  adjoint/forward time, finite differences, Adam fit
   emitters         emitter table (compaction + prefix sums) over 1M triangles
@@ -350,13 +351,14 @@ def two_seed_rel_rmse(img_a: np.ndarray, img_b: np.ndarray) -> float:
     return float(np.sqrt((d * d).sum()) / (np.sqrt(2.0) * np.sqrt((m * m).sum())))
 
 
-def render_convergence(a, scene, label: str, max_spp: int) -> dict:
-    """Wall seconds to 1% rel-RMSE for the PBR path loop: renders of 1, 2, 4,
-    ... spp (seed 7, timed from the call to the image in host memory), each
-    with a second render at seed 8 for the two-seed error estimate, until the
-    estimate is <= 1%."""
+def render_convergence(a, scene, label: str, max_spp: int, mode: int | None = None) -> dict:
+    """Wall seconds to 1% rel-RMSE for the PBR path loop (or, with mode, the
+    compat estimator): renders of 1, 2, 4, ... spp (seed 7, timed from the
+    call to the image in host memory), each with a second render at seed 8
+    for the two-seed error estimate, until the estimate is <= 1%."""
     from aurora_amd import native
 
+    mode = native.MODE_PBR if mode is None else mode
     ctx = native.Context(0)
     ctx.upload_scene(scene)
     ctx.rebuild_nmh()
@@ -365,8 +367,7 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
     w, h = scene.width, scene.height
     # warm-up: code objects and the largest path buffers
     ctx.frame_create(w, h, 64)
-    ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, 64, scene.max_depth,
-                                                       scene.rr_start, 1))
+    ctx.frame_render(scene.camera, native.Context.opts(mode, 64, scene.max_depth, scene.rr_start, 1))
     ctx.synchronize()
     spp, rows, hit = (a.spp or 1), [], None
     if a.spp:
@@ -374,14 +375,12 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
     while spp <= max_spp:
         ctx.frame_create(w, h, spp)
         t0 = time.perf_counter()
-        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, spp, scene.max_depth,
-                                                           scene.rr_start, 7))
+        ctx.frame_render(scene.camera, native.Context.opts(mode, spp, scene.max_depth, scene.rr_start, 7))
         img_a = ctx.frame_read()
         dt = time.perf_counter() - t0
         rays = ctx.rays_traced()
         rays_ext, rays_sh = ctx.ray_counts()
-        ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, spp, scene.max_depth,
-                                                           scene.rr_start, 8))
+        ctx.frame_render(scene.camera, native.Context.opts(mode, spp, scene.max_depth, scene.rr_start, 8))
         img_b = ctx.frame_read()
         err = two_seed_rel_rmse(img_a, img_b)
         rows.append({"spp": spp, "s": round(dt, 4), "rel_rmse": round(err, 5),
@@ -405,8 +404,8 @@ def render_convergence(a, scene, label: str, max_spp: int) -> dict:
         ctx.frame_create(w, h, part)
         R = np.zeros((w * h, 3))
         for k in range(rs // part):
-            ctx.frame_render(scene.camera, native.Context.opts(native.MODE_PBR, part, scene.max_depth,
-                                                               scene.rr_start, 9 + k))
+            ctx.frame_render(scene.camera, native.Context.opts(mode, part, scene.max_depth, scene.rr_start,
+                                                               9 + k))
             R += ctx.frame_read()[:, :3]
         R /= rs // part
         d = float(np.linalg.norm(img_a[:, :3] - R) / np.linalg.norm(R))
@@ -845,11 +844,22 @@ def compat_frame(a, scene, label: str) -> dict:
             "rel_rmse_two_seed": round(err, 5), "runs": len(times), "render_params": params}
 
 
+def compat_convergence(a, scene, label: str, max_spp: int) -> dict | None:
+    """The compat estimator's seconds to 1% rel-RMSE (render_convergence in
+    compat mode), unless one render at --spp was asked for."""
+    from aurora_amd import native
+
+    if a.spp:
+        return None
+    return render_convergence(a, scene, f"{label} compat", max_spp, mode=native.MODE_COMPAT)
+
+
 def workload_compat_cornell(a) -> dict:
     from aurora_amd import scenes
 
     scene = scenes.cornell()  # 256^2, 16 spp
     r = compat_frame(a, scene, "cornell")
+    r["convergence"] = compat_convergence(a, scene, "cornell", 1 << 14)
     return {"metric": "compat render ms (Cornell box + 2 spheres, 256x256, 16 spp)", "value": r["ms"],
             "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
             "data": "synthetic (seeded Cornell box + 2 icospheres)",
@@ -861,6 +871,7 @@ def workload_compat_spheres(a) -> dict:
 
     scene = scenes.spheres()  # 1920x1080, 64 spp, Phong materials with reflectivity 0.2
     r = compat_frame(a, scene, "spheres")
+    r["convergence"] = compat_convergence(a, scene, "spheres", 4096)
     return {"metric": "compat render ms (200 spheres, 1M triangles, 1920x1080, 64 spp)", "value": r["ms"],
             "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
             "data": "synthetic (seeded 200 icospheres + ground, 4 quad lights)",
