@@ -142,6 +142,53 @@ def test_trace_soup_with_ties(gpu_ctx, oracle):
     assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32))
 
 
+def _degenerate_rays(n: int, seed: int) -> np.ndarray:
+    """Rays whose slab quotients hit the division's special cases: origins
+    on the grid soup's coordinate planes (multiples of 0.25, some jittered),
+    directions along an axis, in a coordinate plane (one component +0.0 or
+    -0.0) or along a plane diagonal, a few general ones, and a few zero
+    vectors."""
+    rng = np.random.default_rng(seed)
+    rays = np.zeros((n, 8), np.float32)
+    o = np.round(rng.uniform(-0.5, 10.5, (n, 3)) / 0.25) * 0.25
+    o = np.where(rng.random((n, 3)) < 0.3, rng.uniform(-0.5, 10.5, (n, 3)), o)
+    rays[:, 0:3] = o
+    rays[:, 3] = 1e6
+    kind = rng.integers(0, 4, n)
+    axis = rng.integers(0, 3, n)
+    sign = np.where(rng.random((n, 3)) < 0.5, -1.0, 1.0)
+    d = rng.normal(size=(n, 3))
+    rows = np.arange(n)
+    along = np.zeros((n, 3))
+    along[rows, axis] = 1.0
+    d = np.where((kind == 0)[:, None], along, d)                      # along an axis
+    d = np.where((kind == 1)[:, None], d * (1.0 - along), d)          # in a coordinate plane
+    d = np.where((kind == 2)[:, None], (1.0 - along) / np.sqrt(2.0), d)  # a plane diagonal
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    d = np.abs(d) * sign  # zeros get both signs
+    d[:16] = 0.0
+    rays[:, 4:7] = d.astype(np.float32)
+    return rays
+
+
+def test_trace_degenerate_directions(gpu_ctx, oracle):
+    """Slab quotients of +-inf and 0/0 = NaN (zero direction components,
+    origins on the slab planes, zero directions): closest and any hits equal
+    the oracle's bit for bit."""
+    s = scenes.random_soup(seed=31, n=20_000, box=10.0, grid=0.25)
+    rays = _degenerate_rays(60_000, seed=32)
+    assert (rays[:, 4:7] == 0.0).any(axis=1).mean() > 0.5
+    ids_g, tuv_g, ids_o, tuv_o, tree_v = _trace_both(gpu_ctx, oracle, s, rays)
+    assert (ids_o >= 0).mean() > 0.05
+    assert np.array_equal(ids_g, ids_o)
+    assert np.array_equal(tuv_g.view(np.uint32), tuv_o.view(np.uint32))
+    shadow = rays.copy()
+    shadow[:, 3] = np.random.default_rng(33).uniform(0.1, 8.0, rays.shape[0]).astype(np.float32)
+    any_g, _ = gpu_ctx.trace(shadow, any_hit=True)
+    any_o, _ = oracle.trace(tree_v, shadow, any_hit=True)
+    assert np.array_equal(any_g, any_o)
+
+
 def test_trace_launch_shapes_agree(gpu_ctx, oracle):
     """Every launch shape (one lane per ray, persistent waves with any refill
     threshold, stack partly in global memory) gives the oracle's hits."""
