@@ -136,6 +136,34 @@ def time_launches(ctx, launch, steps: int, warmup: int):
     return ev[0].elapsed_time(ev[-1]) / steps, per
 
 
+def host_inclusive_trace(ctx, rays_np: np.ndarray, rays, hits, launch) -> dict:
+    """The rate for a caller whose rays and hits live in host memory (the
+    trace entry points take device pointers, so the headline value has no
+    transfer in it): pinned host rays copied in, one launch, the hits copied
+    back, each step synchronised; median of 3 after a warm-up."""
+    import torch
+
+    n = rays_np.shape[0]
+    h_rays = torch.from_numpy(rays_np).pin_memory()
+    h_hits = torch.empty(tuple(hits.shape), dtype=hits.dtype, pin_memory=True)
+    times = []
+    for it in range(4):
+        t0 = time.perf_counter()
+        rays.copy_(h_rays)
+        torch.cuda.synchronize()
+        launch()
+        ctx.synchronize()
+        h_hits.copy_(hits)
+        torch.cuda.synchronize()
+        if it:
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    moved = rays_np.nbytes + hits.numel() * hits.element_size()
+    log(f"host-inclusive: {t * 1e3:.2f} ms ({n / t / 1e6:.0f} Mrays/s; {moved / 1e6:.0f} MB over PCIe)")
+    return {"ms": round(t * 1e3, 3), "mrays": round(n / t / 1e6, 1), "pcie_bytes": int(moved),
+            "note": "pinned host rays in, one launch, hits out; synchronous"}
+
+
 def workload_trace1m(a) -> dict:
     import torch
 
@@ -172,6 +200,7 @@ def workload_trace1m(a) -> dict:
     mrays = a.rays / (ms_step * 1e-3) / 1e6
     achieved = bytes_per_launch / (ms_step * 1e-3) / 1e9
     log(f"{ms_step:.3f} ms/launch (median {np.median(per):.3f}), {mrays:.1f} Mrays/s")
+    host = host_inclusive_trace(ctx, rays_np, rays, hits, launch)
 
     cpu = None
     if not a.no_cpu:
@@ -203,6 +232,7 @@ def workload_trace1m(a) -> dict:
             "passed_per_ray": round(passed / a.rays, 3),
             "hit_rate": round(hit_rate, 4),
             "trace_params": params,
+            "host_inclusive": host,
         },
         "roofline": trace_roofline(bytes_per_launch, ms_step, "k_trace_incoherent_1M.md"),
         "cpu_baseline": cpu,
